@@ -1,0 +1,224 @@
+/*
+ * mini-castro-amd: MI355X (gfx950) CTU hydro kernels behind mini-Castro's
+ * C ABI.
+ *
+ * Part 1 declares, with the same names and argument lists, the extern "C"
+ * entry points a mini-Castro host binds (Source/Castro_F.H:11-161).  A FAB
+ * argument is the triplet (double* p, const int* p_lo, const int* p_hi) of
+ * BL_FORT_FAB_ARG_3D: Fortran order, i fastest, component slowest
+ * (Source/Castro.H:19).  Semantics are those of the reference's OpenACC /
+ * OpenMP-offload builds (Source/Castro.H:23-24): the host calls each entry
+ * point once per box with the whole box as (lo, hi) and DEVICE pointers; the
+ * kernel is launched asynchronously on the stream given to mc_set_stream
+ * (default: the null stream).  lo/hi and the FAB bounds are read during the
+ * call and never retained.  Component maps are those of
+ * Source/Castro_nd.F90:14-57 (1-based there, idir is 1-based here as well).
+ *
+ * Part 2 declares extensions that have no counterpart in the reference: a
+ * stream setter, table loading from an explicit path or from memory, a
+ * problem-constant setter, batch EOS evaluation, and the level driver that
+ * advances a whole box list (the role of Castro::advance) either with the
+ * reference's per-box kernel sequence or with the batched fused path.
+ *
+ * Errors: the reference routines return void and so do the Part 1 entry
+ * points; a failure is printed and recorded for mc_last_error().  Part 2
+ * calls return 0 on success and a negative code on failure.
+ */
+#ifndef MC_CASTRO_H
+#define MC_CASTRO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================== Part 1: Source/Castro_F.H ===================== */
+
+/* Source/Castro_F.H:11-13 (network.F90 has no state: no-ops) */
+void network_init(void);
+void network_finalize(void);
+
+/* Source/Castro_F.H:15-17 -> Source/eos.F90:580-734, 1063-1101.
+   eos_init reads ./helm_table.dat and aborts if it cannot (eos.F90:637-641). */
+void eos_init(void);
+void eos_finalize(void);
+
+/* Source/Castro_F.H:19-23 -> Source/Castro_hydro_nd.F90:73-170 */
+void ctoprim(const int *lo, const int *hi,
+             const double *u, const int *u_lo, const int *u_hi,
+             double *q, const int *q_lo, const int *q_hi,
+             double *qaux, const int *qa_lo, const int *qa_hi);
+
+/* Source/Castro_F.H:25-39 -> Source/Castro_nd.F90:69-297 (clean_state kernels) */
+void enforce_minimum_density(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi);
+void normalize_species(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi);
+void reset_internal_e(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi);
+void compute_temp(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi);
+
+/* Source/Castro_F.H:41-45 -> Source/timestep.F90:12-72.  *dt = min(*dt, CFL
+   time step of the box).  dt may be host, managed or device memory; the call
+   returns after the stream has drained. */
+void estdt(const int *lo, const int *hi,
+           const double *state, const int *state_lo, const int *state_hi,
+           const double *dx, double *dt);
+
+/* Source/Castro_F.H:47-52 -> Source/Castro_hydro_nd.F90:301-340 */
+void store_flux(const int *lo, const int *hi,
+                double *flux_out, const int *flux_out_lo, const int *flux_out_hi,
+                double *flux_in, const int *flux_in_lo, const int *flux_in_hi,
+                double *area, const int *area_lo, const int *area_hi,
+                const double dt);
+
+/* Source/Castro_F.H:54-70 -> Source/Castro_hydro_nd.F90:344-441 */
+void fill_hydro_source(const int *lo, const int *hi,
+                       const double *state, const int *state_lo, const int *state_hi,
+                       const double *q, const int *q_lo, const int *q_hi,
+                       double *source, const int *source_lo, const int *source_hi,
+                       const double *flux0, const int *flux0_lo, const int *flux0_hi,
+                       const double *flux1, const int *flux1_lo, const int *flux1_hi,
+                       const double *flux2, const int *flux2_lo, const int *flux2_hi,
+                       const double *qe0, const int *qe0_lo, const int *qe0_hi,
+                       const double *qe1, const int *qe1_lo, const int *qe1_hi,
+                       const double *qe2, const int *qe2_lo, const int *qe2_hi,
+                       const double *area0, const int *area0_lo, const int *area0_hi,
+                       const double *area1, const int *area1_lo, const int *area1_hi,
+                       const double *area2, const int *area2_lo, const int *area2_hi,
+                       const double *volume, const int *volume_lo, const int *volume_hi,
+                       const double *dx, const double dt);
+
+/* Source/Castro_F.H:72-75 -> Source/Castro_hydro_nd.F90:248-297 */
+void normalize_species_fluxes(const int *lo, const int *hi, double *flux, const int *flux_lo, const int *flux_hi);
+
+/* Source/Castro_F.H:77-83 -> Source/Castro_hydro_nd.F90:174-245 */
+void apply_av(const int *lo, const int *hi, const int idir, const double *dx,
+              double *div, const int *div_lo, const int *div_hi,
+              double *Sborder, const int *Sborder_lo, const int *Sborder_hi,
+              double *flux, const int *flux_lo, const int *flux_hi);
+
+/* Source/Castro_F.H:85-96 -> Source/trans.F90:14-243 */
+void trans1(const int *lo, const int *hi, const int idir1, const int idir2,
+            const double *q2m, const int *q2m_lo, const int *q2m_hi,
+            double *q2mo, const int *q2mo_lo, const int *q2mo_hi,
+            const double *q2p, const int *q2p_lo, const int *q2p_hi,
+            double *q2po, const int *q2po_lo, const int *q2po_hi,
+            const double *qaux, const int *qaux_lo, const int *qaux_hi,
+            const double *f1, const int *f1_lo, const int *f1_hi,
+            const double *q1, const int *q1_lo, const int *q1_hi,
+            const double cdtdx);
+
+/* Source/Castro_F.H:98-111 -> Source/trans.F90:250-505 */
+void trans2(const int *lo, const int *hi, const int idir1, const int idir2, const int idir3,
+            const double *qm, const int *qm_lo, const int *qm_hi,
+            double *qmo, const int *qmo_lo, const int *qmo_hi,
+            const double *qp, const int *qp_lo, const int *qp_hi,
+            double *qpo, const int *qpo_lo, const int *qpo_hi,
+            const double *qaux, const int *qaux_lo, const int *qaux_hi,
+            const double *fyz, const int *fyz_lo, const int *fyz_hi,
+            const double *fzy, const int *fzy_lo, const int *fzy_hi,
+            const double *qy, const int *qy_lo, const int *qy_hi,
+            const double *qz, const int *qz_lo, const int *qz_hi,
+            const double hdtdx, const double cdtdy, const double cdtdz);
+
+/* Source/Castro_F.H:113-121 -> Source/riemann.F90:10-331 */
+void compute_flux(const int *lo, const int *hi,
+                  const double *ql, const int *ql_lo, const int *ql_hi,
+                  const double *qr, const int *qr_lo, const int *qr_hi,
+                  double *flux, const int *flux_lo, const int *flux_hi,
+                  double *q_int, const int *q_int_lo, const int *q_int_hi,
+                  double *qe, const int *qe_lo, const int *qe_hi,
+                  const double *qaux, const int *qaux_lo, const int *qaux_hi,
+                  const int idir);
+
+/* Source/Castro_F.H:123-127 -> Source/Castro_hydro_nd.F90:11-69 */
+void divu(const int *lo, const int *hi,
+          double *q, const int *q_lo, const int *q_hi,
+          const double *dx,
+          double *div, const int *div_lo, const int *div_hi);
+
+/* Source/Castro_F.H:129-140 -> Source/ppm.F90:309-743 */
+void trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi, const int idir,
+               const double *q, const int *q_lo, const int *q_hi,
+               const double *qaux, const int *qaux_lo, const int *qaux_hi,
+               double *qm, const int *qm_lo, const int *qm_hi,
+               double *qp, const int *qp_lo, const int *qp_hi,
+               const int *domlo, const int *domhi, const double *dx, const double dt);
+
+/* Source/Castro_F.H:142-146 -> Source/Prob.F90:19-129; the problem
+   constants (Prob.F90:11-15) come from mc_set_problem, defaulting to the
+   reference's values. */
+void initdata(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi,
+              const double *dx, const double *problo, const double *probhi);
+
+/* Source/Castro_F.H:148-152 -> Source/Castro_nd.F90:301-345 */
+void denerror(const int *lo, const int *hi,
+              int8_t *tag, const int *tag_lo, const int *tag_hi,
+              const double *den, const int *den_lo, const int *den_hi,
+              const int8_t tagval, const int8_t clearval);
+
+/* Source/Castro_F.H:154-159 -> Source/Castro_nd.F90:349-400.  blast_mass and
+   blast_radius are host or managed memory and are accumulated into; the
+   call synchronises. */
+void calculate_blast_radius(const int *lo, const int *hi,
+                            double *state, const int *state_lo, const int *state_hi,
+                            const double *dx, const double *problo, const double *probhi,
+                            double *blast_mass, double *blast_radius, const double max_density);
+
+/* ===================== Part 2: extensions ===================== */
+
+/* Stream for subsequent Part 1 launches of the calling thread (hipStream_t). */
+void mc_set_stream(void *stream);
+void *mc_get_stream(void);
+/* Text of the last error recorded on the calling thread. */
+const char *mc_last_error(void);
+
+/* Table loading: a helm_table.dat text file or its ".bin" cache; 21 host
+   columns (541 x 201 each, i fastest, column order of Source/eos.F90:82-94);
+   or the table the library generates (ideal electron gas, layout of
+   Source/eos.F90:626-670). */
+int mc_eos_init_from(const char *path);
+int mc_eos_init_arrays(const double *tab21);
+int mc_eos_init_generated(void);
+
+/* Device batch EOS: mode 1 (rho,T), 2 (rho,e), 3 (rho,p); ev holds e or p.
+   out (device, 10 doubles per point): T p e cs gam1 dpde dpdr_e dedT iat jat. */
+int mc_eos_eval_batch(int n, int mode, const double *rho, const double *T, const double *ev,
+                      const double *abar, const double *zbar, double *out);
+
+/* Problem constants used by initdata (Source/Prob.F90:11-15). */
+void mc_set_problem(double p_ambient, double dens_ambient, double exp_energy, double r_init, int nsub);
+
+/* ---- level driver: one AMR level, a list of boxes tiling the domain ---- */
+typedef struct mc_level mc_level_t;
+enum { MC_BC_PERIODIC = 0, MC_BC_OUTFLOW = 1 };
+enum { MC_MODE_PLAIN = 0, MC_MODE_FUSED = 1 };
+
+/* boxes: nbox x 6 ints (lo x,y,z, hi x,y,z) tiling [0, n_cell). */
+mc_level_t *mc_level_create(const int *n_cell, const double *prob_lo, const double *prob_hi, int bc,
+                            int nbox, const int *boxes);
+void mc_level_destroy(mc_level_t *L);
+int mc_level_set_mode(mc_level_t *L, int mode);
+int mc_level_nbox(const mc_level_t *L);
+/* S_new of box b (20 components, no ghosts, Fortran order) */
+double *mc_level_state_ptr(mc_level_t *L, int b);
+int mc_level_set_state(mc_level_t *L, int b, const double *host);
+int mc_level_get_state(mc_level_t *L, int b, double *host);
+/* Castro::initData with initdata on every box (Source/Castro.cpp:80-117) */
+int mc_level_init_data(mc_level_t *L, double p_ambient, double dens_ambient, double exp_energy,
+                       double r_init, int nsub);
+double mc_level_est_time_step(mc_level_t *L);   /* Source/Castro.cpp:175-216 */
+double mc_level_initial_dt(mc_level_t *L);      /* Source/Castro.cpp:162-173 */
+int mc_level_advance(mc_level_t *L, double dt); /* Source/Castro_advance.cpp:11-100 */
+int mc_level_post_timestep(mc_level_t *L);      /* Source/Castro.cpp:343-369 */
+/* Amr::coarseTimeStep: computeInitialDt/computeNewDt, advance,
+   post_timestep.  Returns the dt taken (negative on error). */
+double mc_level_coarse_step(mc_level_t *L, double stop_time);
+int mc_level_steps(const mc_level_t *L);
+double mc_level_time(const mc_level_t *L);
+double mc_level_dt(const mc_level_t *L);
+int mc_level_sync(mc_level_t *L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MC_CASTRO_H */

@@ -1,0 +1,20 @@
+// Library-internal helpers shared by the translation units.
+#pragma once
+
+#include "mc_common.h"
+#include "mc_eos.h"
+
+namespace mc {
+
+hipStream_t cur_stream();
+const EosTable &eos_table();
+bool eos_ready();
+void set_error(const char *msg);
+int check_hip(hipError_t e, const char *what);
+
+// Asynchronous estdt over box bx of a state FAB on sbox: min-reduces the
+// CFL time step of every cell into *dt_dev (no synchronisation).
+void launch_estdt(const Box &bx, const double *state, const Box &sbox, const double *dx, double *dt_dev,
+                  hipStream_t s);
+
+} // namespace mc

@@ -1,0 +1,564 @@
+// Level driver: the role of Castro::advance / construct_hydro_source /
+// clean_state / estTimeStep on one AMR level of boxes (single GPU).
+//
+// Plain mode replays the reference's per-box kernel sequence through the
+// drop-in C ABI (Source/Castro_hydro.cpp:31-397: 36 launches per box, plus
+// the four clean_state kernels at each of its four call sites and estdt).
+// Fused mode hands the whole box list to the batched kernels (mc_fused.hip).
+//
+// Device memory layout: every MultiFab (S_new, S_old, Sborder, hydro
+// source) is one arena holding the boxes' FABs back to back, each in
+// Fortran order with 20 components; the per-box temporaries of the plain
+// path are allocated once for the largest box and reused box after box.
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "../../include/mc_castro.h"
+#include "mc_common.h"
+#include "mc_fused.h"
+#include "mc_internal.h"
+
+namespace mc {
+
+static Box grow(Box b, int g0, int g1, int g2)
+{
+    b.lo[0] -= g0; b.lo[1] -= g1; b.lo[2] -= g2;
+    b.hi[0] += g0; b.hi[1] += g1; b.hi[2] += g2;
+    return b;
+}
+static Box grow1(Box b, int g) { return grow(b, g, g, g); }
+static Box nodes(Box b, int d) { b.hi[d] += 1; return b; }
+
+struct FabDesc {
+    int lo[3], hi[3];
+    long off;  // offset in doubles inside an arena
+};
+
+// FillPatch of Sborder (grown by 4) from S_old: exact copies of the owning
+// box's value; periodic images, or the nearest domain cell (first-order
+// extrapolation) outside the domain.
+__global__ __launch_bounds__(kBlock) void k_fill_patch(BoxIter it, double *dst, long dst_sn, const double *src_arena,
+                                                       const FabDesc *fabs, const int *owner, int gran, int mx,
+                                                       int my, int n0, int n1, int n2, int periodic)
+{
+    int i, j, k;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!iter_cell(it, t, i, j, k)) return;
+    int g0 = i, g1 = j, g2 = k;
+    if (periodic) {
+        g0 = ((g0 % n0) + n0) % n0;
+        g1 = ((g1 % n1) + n1) % n1;
+        g2 = ((g2 % n2) + n2) % n2;
+    } else {
+        g0 = min(max(g0, 0), n0 - 1);
+        g1 = min(max(g1, 0), n1 - 1);
+        g2 = min(max(g2, 0), n2 - 1);
+    }
+    const int ob = owner[(g0 / gran) + mx * ((g1 / gran) + my * (g2 / gran))];
+    const FabDesc f = fabs[ob];
+    const long fx = f.hi[0] - f.lo[0] + 1, fxy = fx * (f.hi[1] - f.lo[1] + 1);
+    const long fn = fxy * (f.hi[2] - f.lo[2] + 1);
+    const double *s = src_arena + f.off + (g0 - f.lo[0]) + fx * (g1 - f.lo[1]) + fxy * (g2 - f.lo[2]);
+#pragma unroll 4
+    for (int c = 0; c < NVAR; ++c) dst[t + dst_sn * c] = s[fn * c];
+}
+
+// MultiFab::Saxpy(S_new, dt, hydro_source): s = s + dt*h
+__global__ __launch_bounds__(kBlock) void k_saxpy(long n, double *s, const double *h, double dt)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) s[t] = s[t] + dt * h[t];
+}
+
+__global__ void k_fill(long n, double *p, double v)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+} // namespace mc
+
+using namespace mc;
+
+struct mc_level {
+    int n_cell[3];
+    double prob_lo[3], prob_hi[3], dx[3];
+    int bc;
+    int mode;
+    std::vector<Box> boxes;
+    std::vector<long> off_valid, off_grown;  // arena offsets (doubles)
+    long n_valid = 0, n_grown = 0;
+    double *S_new = nullptr, *S_old = nullptr, *Sborder = nullptr, *src = nullptr;
+    FabDesc *fabs_dev = nullptr;
+    int *owner_dev = nullptr;
+    int gran = 1, mx = 0, my = 0;
+    // plain-path temporaries, sized for the largest box
+    double *q = nullptr, *qaux = nullptr, *divb = nullptr, *q_int = nullptr, *ftmp1 = nullptr, *ftmp2 = nullptr;
+    double *qg1 = nullptr, *qg2 = nullptr, *ql = nullptr, *qr = nullptr, *vol = nullptr;
+    double *flux[3] = {}, *qe[3] = {}, *area[3] = {}, *fluxes_out[3] = {};
+    double *qm[3][3] = {}, *qp[3][3] = {};
+    double *dt_dev = nullptr;
+    hipStream_t stream = nullptr;
+    int steps = 0;
+    double time = 0.0, dt = 0.0;
+    FusedLevel *fused = nullptr;
+};
+
+static double *dalloc(size_t n, bool &ok)
+{
+    double *p = nullptr;
+    if (check_hip(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)), "hipMalloc(level)")) ok = false;
+    return p;
+}
+
+static void fill_const(hipStream_t s, double *p, long n, double v)
+{
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, n, p, v);
+}
+
+extern "C" {
+
+mc_level_t *mc_level_create(const int *n_cell, const double *prob_lo, const double *prob_hi, int bc, int nbox,
+                            const int *boxes)
+{
+    if (nbox <= 0) {
+        set_error("mc_level_create: no boxes");
+        return nullptr;
+    }
+    mc_level_t *L = new mc_level_t();
+    long ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        L->n_cell[d] = n_cell[d];
+        L->prob_lo[d] = prob_lo[d];
+        L->prob_hi[d] = prob_hi[d];
+        L->dx[d] = (prob_hi[d] - prob_lo[d]) / (double)n_cell[d];
+        ncell *= n_cell[d];
+    }
+    L->bc = bc;
+    L->mode = MC_MODE_PLAIN;
+    long vsum = 0;
+    int g = 0;
+    for (int b = 0; b < nbox; ++b) {
+        Box bx;
+        for (int d = 0; d < 3; ++d) {
+            bx.lo[d] = boxes[6 * b + d];
+            bx.hi[d] = boxes[6 * b + 3 + d];
+            if (bx.lo[d] < 0 || bx.hi[d] >= n_cell[d] || bx.hi[d] < bx.lo[d]) {
+                set_error("mc_level_create: box outside the domain");
+                delete L;
+                return nullptr;
+            }
+            g = std::gcd(g, bx.lo[d]);
+            g = std::gcd(g, bx.hi[d] + 1);
+        }
+        L->boxes.push_back(bx);
+        L->off_valid.push_back(L->n_valid);
+        L->off_grown.push_back(L->n_grown);
+        L->n_valid += box_npts(bx) * NVAR;
+        L->n_grown += box_npts(grow1(bx, 4)) * NVAR;
+        vsum += box_npts(bx);
+    }
+    if (vsum != ncell) {
+        set_error("mc_level_create: boxes do not tile the domain");
+        delete L;
+        return nullptr;
+    }
+    L->gran = g > 0 ? g : 1;
+    for (int d = 0; d < 3; ++d)
+        if (n_cell[d] % L->gran) L->gran = 1;
+    L->mx = n_cell[0] / L->gran;
+    L->my = n_cell[1] / L->gran;
+    const int mz = n_cell[2] / L->gran;
+    std::vector<int> owner((size_t)L->mx * L->my * mz, -1);
+    std::vector<FabDesc> fd(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        const Box &bx = L->boxes[b];
+        for (int d = 0; d < 3; ++d) {
+            fd[b].lo[d] = bx.lo[d];
+            fd[b].hi[d] = bx.hi[d];
+        }
+        fd[b].off = L->off_valid[b];
+        for (int k = bx.lo[2] / L->gran; k <= bx.hi[2] / L->gran; ++k)
+            for (int j = bx.lo[1] / L->gran; j <= bx.hi[1] / L->gran; ++j)
+                for (int i = bx.lo[0] / L->gran; i <= bx.hi[0] / L->gran; ++i) {
+                    int &o = owner[i + (size_t)L->mx * (j + (size_t)L->my * k)];
+                    if (o != -1) {
+                        set_error("mc_level_create: overlapping boxes");
+                        delete L;
+                        return nullptr;
+                    }
+                    o = b;
+                }
+    }
+    bool ok = true;
+    ok = ok && !check_hip(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking), "hipStreamCreate");
+    L->S_new = dalloc(L->n_valid, ok);
+    L->S_old = dalloc(L->n_valid, ok);
+    L->src = dalloc(L->n_valid, ok);
+    L->Sborder = dalloc(L->n_grown, ok);
+    L->dt_dev = dalloc(1, ok);
+    if (ok) ok = !check_hip(hipMalloc(&L->fabs_dev, sizeof(FabDesc) * nbox), "hipMalloc(fabs)");
+    if (ok) ok = !check_hip(hipMalloc(&L->owner_dev, sizeof(int) * owner.size()), "hipMalloc(owner)");
+    if (ok) ok = !check_hip(hipMemcpy(L->fabs_dev, fd.data(), sizeof(FabDesc) * nbox, hipMemcpyHostToDevice), "h2d");
+    if (ok) ok = !check_hip(hipMemcpy(L->owner_dev, owner.data(), sizeof(int) * owner.size(), hipMemcpyHostToDevice), "h2d");
+    if (!ok) {
+        mc_level_destroy(L);
+        return nullptr;
+    }
+    return L;
+}
+
+static void free_plain(mc_level_t *L)
+{
+    double **ps[] = {&L->q, &L->qaux, &L->divb, &L->q_int, &L->ftmp1, &L->ftmp2, &L->qg1, &L->qg2, &L->ql, &L->qr,
+                     &L->vol};
+    for (double **p : ps) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    for (int d = 0; d < 3; ++d) {
+        double **pd[] = {&L->flux[d], &L->qe[d], &L->area[d], &L->fluxes_out[d]};
+        for (double **p : pd) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+        for (int e = 0; e < 3; ++e) {
+            if (L->qm[d][e]) (void)hipFree(L->qm[d][e]);
+            if (L->qp[d][e]) (void)hipFree(L->qp[d][e]);
+            L->qm[d][e] = L->qp[d][e] = nullptr;
+        }
+    }
+}
+
+// Per-box temporaries of Source/Castro_hydro.cpp:54-170 for the largest box.
+static int alloc_plain(mc_level_t *L)
+{
+    if (L->q) return 0;
+    long mv = 0;
+    int mlen[3] = {0, 0, 0};
+    for (const Box &b : L->boxes)
+        for (int d = 0; d < 3; ++d) mlen[d] = std::max(mlen[d], b.hi[d] - b.lo[d] + 1);
+    Box m;
+    for (int d = 0; d < 3; ++d) {
+        m.lo[d] = 0;
+        m.hi[d] = mlen[d] - 1;
+    }
+    (void)mv;
+    const long nq = box_npts(grow1(m, 4)), no = box_npts(grow1(m, 1));
+    bool ok = true;
+    L->q = dalloc(nq * QVAR, ok);
+    L->qaux = dalloc(nq * NQAUX, ok);
+    L->divb = dalloc(no, ok);
+    L->q_int = dalloc(no * QVAR, ok);
+    L->ftmp1 = dalloc(no * NVAR, ok);
+    L->ftmp2 = dalloc(no * NVAR, ok);
+    L->qg1 = dalloc(no * NGDNV, ok);
+    L->qg2 = dalloc(no * NGDNV, ok);
+    L->ql = dalloc(no * QVAR, ok);
+    L->qr = dalloc(no * QVAR, ok);
+    L->vol = dalloc(nq, ok);
+    for (int d = 0; d < 3; ++d) {
+        const long ne = box_npts(grow1(nodes(m, d), 1));
+        L->flux[d] = dalloc(ne * NVAR, ok);
+        L->qe[d] = dalloc(ne * NGDNV, ok);
+        L->area[d] = dalloc(box_npts(grow1(nodes(m, d), 4)), ok);
+        L->fluxes_out[d] = dalloc(box_npts(nodes(m, d)) * NVAR, ok);
+        for (int e = 0; e < 3; ++e) {
+            // tbx[d][e] is contained in grow(ebx[d], 1)
+            L->qm[d][e] = dalloc(ne * QVAR, ok);
+            L->qp[d][e] = dalloc(ne * QVAR, ok);
+        }
+    }
+    if (!ok) {
+        free_plain(L);
+        return -1;
+    }
+    // Geometry::GetVolume / GetFaceArea, Cartesian
+    const double *dx = L->dx;
+    fill_const(L->stream, L->vol, nq, dx[0] * dx[1] * dx[2]);
+    const double ar[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
+    for (int d = 0; d < 3; ++d) fill_const(L->stream, L->area[d], box_npts(grow1(nodes(m, d), 4)), ar[d]);
+    return check_hip(hipGetLastError(), "alloc_plain fill");
+}
+
+void mc_level_destroy(mc_level_t *L)
+{
+    if (!L) return;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    if (L->fused) fused_destroy(L->fused);
+    free_plain(L);
+    double *ps[] = {L->S_new, L->S_old, L->src, L->Sborder, L->dt_dev};
+    for (double *p : ps)
+        if (p) (void)hipFree(p);
+    if (L->fabs_dev) (void)hipFree(L->fabs_dev);
+    if (L->owner_dev) (void)hipFree(L->owner_dev);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    delete L;
+}
+
+int mc_level_set_mode(mc_level_t *L, int mode)
+{
+    if (mode != MC_MODE_PLAIN && mode != MC_MODE_FUSED) {
+        set_error("mc_level_set_mode: unknown mode");
+        return -1;
+    }
+    L->mode = mode;
+    return 0;
+}
+
+int mc_level_nbox(const mc_level_t *L) { return (int)L->boxes.size(); }
+
+double *mc_level_state_ptr(mc_level_t *L, int b) { return L->S_new + L->off_valid[b]; }
+
+int mc_level_set_state(mc_level_t *L, int b, const double *host)
+{
+    if (b < 0 || b >= (int)L->boxes.size()) return -1;
+    const long n = box_npts(L->boxes[b]) * NVAR;
+    if (check_hip(hipMemcpyAsync(L->S_new + L->off_valid[b], host, n * sizeof(double), hipMemcpyHostToDevice,
+                                 L->stream), "set_state"))
+        return -1;
+    return check_hip(hipStreamSynchronize(L->stream), "set_state sync");
+}
+
+int mc_level_get_state(mc_level_t *L, int b, double *host)
+{
+    if (b < 0 || b >= (int)L->boxes.size()) return -1;
+    const long n = box_npts(L->boxes[b]) * NVAR;
+    if (check_hip(hipMemcpyAsync(host, L->S_new + L->off_valid[b], n * sizeof(double), hipMemcpyDeviceToHost,
+                                 L->stream), "get_state"))
+        return -1;
+    return check_hip(hipStreamSynchronize(L->stream), "get_state sync");
+}
+
+int mc_level_sync(mc_level_t *L) { return check_hip(hipStreamSynchronize(L->stream), "level sync"); }
+
+int mc_level_init_data(mc_level_t *L, double p_ambient, double dens_ambient, double exp_energy, double r_init,
+                       int nsub)
+{
+    if (!eos_ready()) {
+        set_error("mc_level_init_data: EOS table not initialised");
+        return -1;
+    }
+    mc_set_problem(p_ambient, dens_ambient, exp_energy, r_init, nsub);
+    void *prev = mc_get_stream();
+    mc_set_stream(L->stream);
+    fill_const(L->stream, L->S_new, L->n_valid, 0.0); /* S_new.setVal(0.), Castro.cpp:93 */
+    for (size_t b = 0; b < L->boxes.size(); ++b) {
+        const Box &bx = L->boxes[b];
+        initdata(bx.lo, bx.hi, L->S_new + L->off_valid[b], bx.lo, bx.hi, L->dx, L->prob_lo, L->prob_hi);
+    }
+    mc_set_stream(prev);
+    L->steps = 0;
+    L->time = 0.0;
+    L->dt = 0.0;
+    return check_hip(hipStreamSynchronize(L->stream), "init_data");
+}
+
+// Castro::clean_state over every box of an arena (ng ghost cells).
+static void clean_state(mc_level_t *L, double *arena, const std::vector<long> &off, int ng)
+{
+    for (size_t b = 0; b < L->boxes.size(); ++b) {
+        const Box fb = grow1(L->boxes[b], ng);
+        double *s = arena + off[b];
+        enforce_minimum_density(fb.lo, fb.hi, s, fb.lo, fb.hi);
+        normalize_species(fb.lo, fb.hi, s, fb.lo, fb.hi);
+        reset_internal_e(fb.lo, fb.hi, s, fb.lo, fb.hi);
+        compute_temp(fb.lo, fb.hi, s, fb.lo, fb.hi);
+    }
+}
+
+static void fill_patch(mc_level_t *L)
+{
+    for (size_t b = 0; b < L->boxes.size(); ++b) {
+        const Box gb = grow1(L->boxes[b], 4);
+        BoxIter it = make_iter(gb);
+        hipLaunchKernelGGL(k_fill_patch, dim3(grid_for(it.n)), dim3(kBlock), 0, L->stream, it,
+                           L->Sborder + L->off_grown[b], (long)it.n, L->S_old, L->fabs_dev, L->owner_dev, L->gran,
+                           L->mx, L->my, L->n_cell[0], L->n_cell[1], L->n_cell[2], L->bc == MC_BC_PERIODIC);
+    }
+}
+
+// Source/Castro_hydro.cpp:31-397 for one box, through the C ABI.
+static void hydro_box(mc_level_t *L, int ib, double dt)
+{
+    const double *dx = L->dx;
+    const int domlo[3] = {0, 0, 0};
+    const int domhi[3] = {L->n_cell[0] - 1, L->n_cell[1] - 1, L->n_cell[2] - 1};
+    const Box bx = L->boxes[ib];
+    const Box fb = grow1(bx, 4);
+    double *state = L->Sborder + L->off_grown[ib];
+    double *source = L->src + L->off_valid[ib];
+    const Box obx = grow1(bx, 1), qbx = grow1(bx, 4), volb = grow1(bx, 4);
+    Box ebx[3], gebx[3], tbx[3][3], areab[3];
+    for (int d = 0; d < 3; ++d) {
+        ebx[d] = nodes(bx, d);
+        gebx[d] = grow1(ebx[d], 1);
+        areab[d] = grow1(nodes(bx, d), 4);
+    }
+    tbx[0][0] = grow(ebx[0], 0, 1, 1);
+    tbx[0][1] = grow(ebx[0], 0, 0, 1);
+    tbx[0][2] = grow(ebx[0], 0, 1, 0);
+    tbx[1][0] = grow(ebx[1], 0, 0, 1);
+    tbx[1][1] = grow(ebx[1], 1, 0, 1);
+    tbx[1][2] = grow(ebx[1], 1, 0, 0);
+    tbx[2][0] = grow(ebx[2], 0, 1, 0);
+    tbx[2][1] = grow(ebx[2], 1, 0, 0);
+    tbx[2][2] = grow(ebx[2], 1, 1, 0);
+    double *q = L->q, *qaux = L->qaux;
+
+    ctoprim(qbx.lo, qbx.hi, state, fb.lo, fb.hi, q, qbx.lo, qbx.hi, qaux, qbx.lo, qbx.hi);
+    divu(obx.lo, obx.hi, q, qbx.lo, qbx.hi, dx, L->divb, obx.lo, obx.hi);
+
+    const double hdtdx[3] = {0.5 * dt / dx[0], 0.5 * dt / dx[1], 0.5 * dt / dx[2]};
+    const double cdtdx[3] = {dt / dx[0] / 3.0, dt / dx[1] / 3.0, dt / dx[2] / 3.0};
+
+    for (int d = 0; d < 3; ++d)
+        trace_ppm(obx.lo, obx.hi, bx.lo, bx.hi, d + 1, q, qbx.lo, qbx.hi, qaux, qbx.lo, qbx.hi, L->qm[d][d],
+                  tbx[d][d].lo, tbx[d][d].hi, L->qp[d][d], tbx[d][d].lo, tbx[d][d].hi, domlo, domhi, dx, dt);
+
+    for (int d = 0; d < 3; ++d) {
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        compute_flux(tbx[d][d].lo, tbx[d][d].hi, L->qm[d][d], tbx[d][d].lo, tbx[d][d].hi, L->qp[d][d], tbx[d][d].lo,
+                     tbx[d][d].hi, L->ftmp1, obx.lo, obx.hi, L->q_int, obx.lo, obx.hi, L->qg1, obx.lo, obx.hi, qaux,
+                     qbx.lo, qbx.hi, d + 1);
+        const int ts[2] = {t1, t2};
+        for (int s = 0; s < 2; ++s) {
+            const int t = ts[s];
+            trans1(tbx[t][d].lo, tbx[t][d].hi, d + 1, t + 1, L->qm[t][t], tbx[t][t].lo, tbx[t][t].hi, L->qm[t][d],
+                   tbx[t][d].lo, tbx[t][d].hi, L->qp[t][t], tbx[t][t].lo, tbx[t][t].hi, L->qp[t][d], tbx[t][d].lo,
+                   tbx[t][d].hi, qaux, qbx.lo, qbx.hi, L->ftmp1, obx.lo, obx.hi, L->qg1, obx.lo, obx.hi, cdtdx[d]);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        compute_flux(tbx[t1][t2].lo, tbx[t1][t2].hi, L->qm[t1][t2], tbx[t1][t2].lo, tbx[t1][t2].hi, L->qp[t1][t2],
+                     tbx[t1][t2].lo, tbx[t1][t2].hi, L->ftmp1, obx.lo, obx.hi, L->q_int, obx.lo, obx.hi, L->qg1,
+                     obx.lo, obx.hi, qaux, qbx.lo, qbx.hi, t1 + 1);
+        compute_flux(tbx[t2][t1].lo, tbx[t2][t1].hi, L->qm[t2][t1], tbx[t2][t1].lo, tbx[t2][t1].hi, L->qp[t2][t1],
+                     tbx[t2][t1].lo, tbx[t2][t1].hi, L->ftmp2, obx.lo, obx.hi, L->q_int, obx.lo, obx.hi, L->qg2,
+                     obx.lo, obx.hi, qaux, qbx.lo, qbx.hi, t2 + 1);
+        trans2(ebx[d].lo, ebx[d].hi, d + 1, t1 + 1, t2 + 1, L->qm[d][d], tbx[d][d].lo, tbx[d][d].hi, L->ql, obx.lo,
+               obx.hi, L->qp[d][d], tbx[d][d].lo, tbx[d][d].hi, L->qr, obx.lo, obx.hi, qaux, qbx.lo, qbx.hi, L->ftmp1,
+               obx.lo, obx.hi, L->ftmp2, obx.lo, obx.hi, L->qg1, obx.lo, obx.hi, L->qg2, obx.lo, obx.hi, hdtdx[d],
+               hdtdx[t1], hdtdx[t2]);
+        compute_flux(ebx[d].lo, ebx[d].hi, L->ql, obx.lo, obx.hi, L->qr, obx.lo, obx.hi, L->flux[d], gebx[d].lo,
+                     gebx[d].hi, L->q_int, obx.lo, obx.hi, L->qe[d], gebx[d].lo, gebx[d].hi, qaux, qbx.lo, qbx.hi,
+                     d + 1);
+    }
+    for (int d = 0; d < 3; ++d) {
+        apply_av(ebx[d].lo, ebx[d].hi, d + 1, dx, L->divb, obx.lo, obx.hi, state, fb.lo, fb.hi, L->flux[d],
+                 gebx[d].lo, gebx[d].hi);
+        normalize_species_fluxes(ebx[d].lo, ebx[d].hi, L->flux[d], gebx[d].lo, gebx[d].hi);
+        store_flux(ebx[d].lo, ebx[d].hi, L->fluxes_out[d], ebx[d].lo, ebx[d].hi, L->flux[d], gebx[d].lo, gebx[d].hi,
+                   L->area[d], areab[d].lo, areab[d].hi, dt);
+    }
+    fill_hydro_source(bx.lo, bx.hi, state, fb.lo, fb.hi, q, qbx.lo, qbx.hi, source, bx.lo, bx.hi, L->flux[0],
+                      gebx[0].lo, gebx[0].hi, L->flux[1], gebx[1].lo, gebx[1].hi, L->flux[2], gebx[2].lo, gebx[2].hi,
+                      L->qe[0], gebx[0].lo, gebx[0].hi, L->qe[1], gebx[1].lo, gebx[1].hi, L->qe[2], gebx[2].lo,
+                      gebx[2].hi, L->area[0], areab[0].lo, areab[0].hi, L->area[1], areab[1].lo, areab[1].hi,
+                      L->area[2], areab[2].lo, areab[2].hi, L->vol, volb.lo, volb.hi, dx, dt);
+}
+
+double mc_level_est_time_step(mc_level_t *L)
+{
+    if (!eos_ready()) {
+        set_error("mc_level_est_time_step: EOS table not initialised");
+        return -1.0;
+    }
+    double dt = DBL_MAX;
+    if (L->mode == MC_MODE_FUSED) {
+        if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1.0;
+        if (fused_est_dt(L->fused, L->S_new, L->dt_dev, L->stream)) return -1.0;
+    } else {
+        fill_const(L->stream, L->dt_dev, 1, DBL_MAX);
+        for (size_t b = 0; b < L->boxes.size(); ++b)
+            launch_estdt(L->boxes[b], L->S_new + L->off_valid[b], L->boxes[b], L->dx, L->dt_dev, L->stream);
+    }
+    if (check_hip(hipMemcpyAsync(&dt, L->dt_dev, sizeof(double), hipMemcpyDeviceToHost, L->stream), "dt d2h") ||
+        check_hip(hipStreamSynchronize(L->stream), "dt sync"))
+        return -1.0;
+    dt *= cfl; /* Castro.cpp:212-213 */
+    return dt;
+}
+
+double mc_level_initial_dt(mc_level_t *L)
+{
+    const double init_shrink = 0.01;
+    const double e = mc_level_est_time_step(L);
+    return e < 0 ? e : init_shrink * e;
+}
+
+int mc_level_advance(mc_level_t *L, double dt)
+{
+    if (!eos_ready()) {
+        set_error("mc_level_advance: EOS table not initialised");
+        return -1;
+    }
+    std::swap(L->S_old, L->S_new); /* swapTimeLevels, Castro_advance.cpp:32-33 */
+    if (L->mode == MC_MODE_FUSED) {
+        if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+        return fused_advance(L->fused, L->S_old, L->S_new, dt, L->stream);
+    }
+    if (alloc_plain(L)) return -1;
+    void *prev = mc_get_stream();
+    mc_set_stream(L->stream);
+    clean_state(L, L->S_old, L->off_valid, 0); /* :51 */
+    (void)check_hip(hipMemcpyAsync(L->S_new, L->S_old, L->n_valid * sizeof(double), hipMemcpyDeviceToDevice,
+                                   L->stream), "Copy"); /* :55 */
+    fill_patch(L);                                 /* :62 */
+    clean_state(L, L->Sborder, L->off_grown, 4);   /* :66 */
+    fill_const(L->stream, L->src, L->n_valid, 0.0); /* hydro_source.setVal(0), Castro_hydro.cpp:17 */
+    for (size_t b = 0; b < L->boxes.size(); ++b) hydro_box(L, (int)b, dt); /* :70 */
+    hipLaunchKernelGGL(k_saxpy, dim3(grid_for(L->n_valid)), dim3(kBlock), 0, L->stream, L->n_valid, L->S_new, L->src,
+                       dt); /* :74 */
+    clean_state(L, L->S_new, L->off_valid, 0); /* :78 */
+    mc_set_stream(prev);
+    return check_hip(hipGetLastError(), "advance");
+}
+
+int mc_level_post_timestep(mc_level_t *L)
+{
+    if (L->mode == MC_MODE_FUSED) return fused_clean(L->fused, L->S_new, L->stream);
+    void *prev = mc_get_stream();
+    mc_set_stream(L->stream);
+    clean_state(L, L->S_new, L->off_valid, 0); /* Castro.cpp:369 */
+    mc_set_stream(prev);
+    return check_hip(hipGetLastError(), "post_timestep");
+}
+
+double mc_level_coarse_step(mc_level_t *L, double stop_time)
+{
+    double dt_0;
+    if (L->steps > 0) { /* Castro.cpp:218-295 */
+        const double change_max = 1.1e0;
+        double dt_min = mc_level_est_time_step(L);
+        if (dt_min < 0) return -1.0;
+        dt_min = fmin(dt_min, change_max * L->dt);
+        dt_0 = fmin(1.0e+100, dt_min);
+    } else { /* Castro.cpp:297-341 */
+        const double d0 = mc_level_initial_dt(L);
+        if (d0 < 0) return -1.0;
+        dt_0 = fmin(1.0e+100, d0);
+    }
+    const double eps = 0.001 * dt_0;
+    if (stop_time >= 0.0)
+        if ((L->time + dt_0) > (stop_time - eps)) dt_0 = stop_time - L->time;
+    L->dt = dt_0;
+    if (mc_level_advance(L, L->dt)) return -1.0;
+    if (mc_level_post_timestep(L)) return -1.0;
+    L->time += L->dt;
+    L->steps += 1;
+    return L->dt;
+}
+
+int mc_level_steps(const mc_level_t *L) { return L->steps; }
+double mc_level_time(const mc_level_t *L) { return L->time; }
+double mc_level_dt(const mc_level_t *L) { return L->dt; }
+
+} // extern "C"

@@ -1,0 +1,101 @@
+"""One AMR level of boxes advanced on the GPU through the C ABI
+(mc_level_*: the role of Castro::advance / coarseTimeStep)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib, check, _dp, _ip
+
+MODE_PLAIN, MODE_FUSED = 0, 1
+BC_PERIODIC, BC_OUTFLOW = 0, 1
+NVAR = 20
+
+
+class Level:
+    def __init__(self, n_cell, boxes, prob_lo=(0.0, 0.0, 0.0), prob_hi=(1.0, 1.0, 1.0), bc=BC_OUTFLOW,
+                 mode=MODE_PLAIN):
+        self.L = lib()
+        self.n_cell = tuple(int(x) for x in n_cell)
+        self.boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        nc = (C.c_int * 3)(*self.n_cell)
+        lo = (C.c_double * 3)(*prob_lo)
+        hi = (C.c_double * 3)(*prob_hi)
+        self.h = self.L.mc_level_create(nc, lo, hi, int(bc), len(self.boxes),
+                                        self.boxes.ctypes.data_as(_ip))
+        if not self.h:
+            raise RuntimeError("mc_level_create failed: %s" % self.L.mc_last_error().decode())
+        self.set_mode(mode)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mc_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mode(self, mode):
+        check(self.L.mc_level_set_mode(self.h, int(mode)), "mc_level_set_mode")
+        self.mode = mode
+
+    def init_data(self, p_ambient, dens_ambient, exp_energy, r_init, nsub=10, **_):
+        check(self.L.mc_level_init_data(self.h, p_ambient, dens_ambient, exp_energy, r_init, int(nsub)),
+              "mc_level_init_data")
+
+    def _shape(self, b):
+        lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+        return tuple(int(x) for x in (hi - lo + 1)) + (NVAR,)
+
+    def box_state(self, b):
+        a = np.zeros(self._shape(b), order="F")
+        check(self.L.mc_level_get_state(self.h, b, a.ctypes.data_as(_dp)), "mc_level_get_state")
+        return a
+
+    def state(self):
+        out = np.zeros(self.n_cell + (NVAR,), order="F")
+        for b in range(len(self.boxes)):
+            lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+            out[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1, :] = self.box_state(b)
+        return out
+
+    def set_state(self, S):
+        for b in range(len(self.boxes)):
+            lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+            a = np.asfortranarray(S[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1, :])
+            check(self.L.mc_level_set_state(self.h, b, a.ctypes.data_as(_dp)), "mc_level_set_state")
+
+    def est_time_step(self):
+        v = self.L.mc_level_est_time_step(self.h)
+        if v < 0:
+            raise RuntimeError("mc_level_est_time_step: %s" % self.L.mc_last_error().decode())
+        return v
+
+    def advance(self, dt):
+        check(self.L.mc_level_advance(self.h, dt), "mc_level_advance")
+
+    def post_timestep(self):
+        check(self.L.mc_level_post_timestep(self.h), "mc_level_post_timestep")
+
+    def coarse_step(self, stop_time=-1.0):
+        v = self.L.mc_level_coarse_step(self.h, stop_time)
+        if v < 0:
+            raise RuntimeError("mc_level_coarse_step: %s" % self.L.mc_last_error().decode())
+        return v
+
+    def sync(self):
+        check(self.L.mc_level_sync(self.h), "mc_level_sync")
+
+    @property
+    def steps(self):
+        return self.L.mc_level_steps(self.h)
+
+    @property
+    def time(self):
+        return self.L.mc_level_time(self.h)
+
+    @property
+    def dt(self):
+        return self.L.mc_level_dt(self.h)

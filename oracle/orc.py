@@ -1,0 +1,227 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of the CPU restatement of mini-Castro's CTU path
+(``oracle/build/liboracle.so``, built by ``oracle/Makefile``).  Only the tests,
+the smoke check and the benchmark's CPU-baseline leg import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+NVAR, QVAR, NQAUX, NGDNV, NSPEC = 20, 21, 4, 6, 13
+BC_PERIODIC, BC_OUTFLOW = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class EosT(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "rho T p e h s abar zbar dpdT dpdr dedT dedr dhdT dhdr dsdT dsdr dpde dpdr_e "
+        "cv cp xne xnp eta pele ppos mu mu_e y_e gam1 cs dpdA dpdZ dedA dedZ").split()] + [
+        ("iat", C.c_int), ("jat", C.c_int), ("niter", C.c_int), ("pad_", C.c_int)]
+
+
+class ProbT(C.Structure):
+    _fields_ = [("p_ambient", C.c_double), ("dens_ambient", C.c_double),
+                ("exp_energy", C.c_double), ("r_init", C.c_double),
+                ("nsub", C.c_int), ("pad_", C.c_int)]
+
+
+def build():
+    """Compile the oracle (make in oracle/)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_level_create.restype = C.c_void_p
+        L.orc_level_state.restype = _dp
+        L.orc_level_source.restype = _dp
+        L.orc_level_est_time_step.restype = C.c_double
+        L.orc_level_initial_dt.restype = C.c_double
+        L.orc_level_coarse_step.restype = C.c_double
+        L.orc_level_time.restype = C.c_double
+        L.orc_level_dt.restype = C.c_double
+        for f in ("orc_level_state", "orc_level_source", "orc_level_est_time_step",
+                  "orc_level_initial_dt", "orc_level_coarse_step", "orc_level_time",
+                  "orc_level_dt", "orc_level_nbox", "orc_level_steps", "orc_level_destroy",
+                  "orc_level_init_data", "orc_level_advance", "orc_level_post_timestep",
+                  "orc_level_clean_state_new"):
+            getattr(L, f).argtypes = [C.c_void_p] + ([C.c_int] if f in ("orc_level_state", "orc_level_source") else [])
+        L.orc_level_coarse_step.argtypes = [C.c_void_p, C.c_double]
+        L.orc_level_advance.argtypes = [C.c_void_p, C.c_double]
+        L.orc_level_init_data.argtypes = [C.c_void_p, C.POINTER(ProbT)]
+        L.orc_eos.argtypes = [C.c_int, C.POINTER(EosT)]
+        L.orc_eos_init.argtypes = [C.c_char_p]
+        L.orc_ppm_reconstruct.argtypes = [_dp, C.c_double, _dp, _dp]
+        L.orc_ppm_int_profile.argtypes = [C.c_double] * 6 + [_dp, _dp]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.float64 and a.flags["F_CONTIGUOUS"] or a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iv(*v):
+    return (C.c_int * 3)(*v)
+
+
+def dv(*v):
+    return (C.c_double * 3)(*v)
+
+
+# ---------------------------------------------------------------- EOS
+_eos_loaded = None
+
+
+def eos_init(path):
+    global _eos_loaded
+    if _eos_loaded != path:
+        rc = lib().orc_eos_init(path.encode())
+        if rc != 0:
+            raise RuntimeError("orc_eos_init(%s) failed: %d" % (path, rc))
+        _eos_loaded = path
+
+
+def eos_init_arrays(tab):
+    """tab: (21, 201, 541) float64 (column, j, i)."""
+    global _eos_loaded
+    tab = np.ascontiguousarray(tab, dtype=np.float64)
+    lib().orc_eos_init_arrays(ptr(tab))
+    _eos_loaded = None
+
+
+def eos_grid():
+    t = np.zeros(201)
+    d = np.zeros(541)
+    lib().orc_eos_grid(ptr(t), ptr(d))
+    return t, d
+
+
+EOS_RT, EOS_RE, EOS_RP = 1, 2, 3
+
+
+def eos(mode, rho, T, abar, zbar, e=0.0, p=0.0):
+    st = EosT()
+    st.rho, st.T, st.abar, st.zbar, st.e, st.p = rho, T, abar, zbar, e, p
+    lib().orc_eos(mode, C.byref(st))
+    return st
+
+
+# ---------------------------------------------------------------- FABs
+class Fab:
+    """Fortran-ordered array on box [lo, hi] with nc components."""
+
+    def __init__(self, lo, hi, nc, data=None):
+        self.lo = tuple(int(x) for x in lo)
+        self.hi = tuple(int(x) for x in hi)
+        self.nc = nc
+        shape = tuple(h - l + 1 for l, h in zip(self.lo, self.hi)) + (nc,)
+        if data is None:
+            self.a = np.zeros(shape, dtype=np.float64, order="F")
+        else:
+            self.a = np.asfortranarray(np.asarray(data, dtype=np.float64).reshape(shape, order="F"))
+
+    def args(self):
+        return (self.a.ctypes.data_as(_dp), iv(*self.lo), iv(*self.hi))
+
+
+def call(name, *args):
+    """Call an oracle kernel; Fab arguments expand to (ptr, lo, hi)."""
+    fl = []
+    for a in args:
+        if isinstance(a, Fab):
+            fl.extend(a.args())
+        elif isinstance(a, float):
+            fl.append(C.c_double(a))
+        elif isinstance(a, (int, np.integer)):
+            fl.append(C.c_int(int(a)))
+        else:
+            fl.append(a)
+    getattr(lib(), "orc_" + name)(*fl)
+
+
+# ---------------------------------------------------------------- level
+class Level:
+    def __init__(self, n_cell, prob_lo, prob_hi, bc, boxes, nthreads=1):
+        self.L = lib()
+        boxes = np.asarray(boxes, dtype=np.int32).reshape(-1, 6)
+        self.boxes = boxes
+        bx = np.ascontiguousarray(boxes.ravel())
+        self.h = self.L.orc_level_create(iv(*n_cell), dv(*prob_lo), dv(*prob_hi), int(bc),
+                                         len(boxes), bx.ctypes.data_as(_ip), int(nthreads))
+        self.n_cell = tuple(n_cell)
+
+    def __del__(self):
+        try:
+            self.L.orc_level_destroy(self.h)
+        except Exception:
+            pass
+
+    def init_data(self, p_ambient, dens_ambient, exp_energy, r_init, nsub=10):
+        pr = ProbT(p_ambient, dens_ambient, exp_energy, r_init, nsub, 0)
+        self.L.orc_level_init_data(self.h, C.byref(pr))
+
+    def box_state(self, b):
+        lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+        n = int(np.prod(hi - lo + 1)) * NVAR
+        p = self.L.orc_level_state(self.h, b)
+        return np.ctypeslib.as_array(p, shape=(n,))
+
+    def state(self):
+        """Whole-domain state, shape (nx, ny, nz, NVAR) (Fortran order)."""
+        out = np.zeros(self.n_cell + (NVAR,), order="F")
+        for b in range(len(self.boxes)):
+            lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+            shp = tuple(hi - lo + 1) + (NVAR,)
+            out[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1, :] = \
+                self.box_state(b).reshape(shp, order="F")
+        return out
+
+    def set_state(self, S):
+        for b in range(len(self.boxes)):
+            lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+            v = self.box_state(b)
+            v[:] = np.asfortranarray(S[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1, :]).ravel(order="F")
+
+    def est_time_step(self):
+        return self.L.orc_level_est_time_step(self.h)
+
+    def initial_dt(self):
+        return self.L.orc_level_initial_dt(self.h)
+
+    def advance(self, dt):
+        self.L.orc_level_advance(self.h, dt)
+
+    def post_timestep(self):
+        self.L.orc_level_post_timestep(self.h)
+
+    def coarse_step(self, stop_time=-1.0):
+        return self.L.orc_level_coarse_step(self.h, stop_time)
+
+    @property
+    def steps(self):
+        return self.L.orc_level_steps(self.h)
+
+    @property
+    def time(self):
+        return self.L.orc_level_time(self.h)
+
+    @property
+    def dt(self):
+        return self.L.orc_level_dt(self.h)

@@ -36,13 +36,11 @@ def main():
     mode = mc.MODE_FUSED if a.mode == "fused" else mc.MODE_PLAIN
     L = mc.Level(cfg["n_cell"], cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mode)
     L.init_data(**cfg)
-    for _ in range(a.warmup):
-        L.coarse_step()
+    L.run(a.warmup)
     L.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        L.coarse_step()
+    L.run(a.steps)
     L.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()

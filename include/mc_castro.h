@@ -188,7 +188,11 @@ int mc_eos_eval_batch(int n, int mode, const double *rho, const double *T, const
 /* Problem constants used by initdata (Source/Prob.F90:11-15). */
 void mc_set_problem(double p_ambient, double dens_ambient, double exp_energy, double r_init, int nsub);
 
-/* ---- level driver: one AMR level, a list of boxes tiling the domain ---- */
+/* ---- level driver: one AMR level, a list of boxes tiling the domain ----
+   MC_MODE_PLAIN replays the reference's per-box kernel sequence through the
+   Part 1 entry points; MC_MODE_FUSED advances all boxes with one launch per
+   stage (bit-identical results).  In fused mode mc_level_advance includes
+   post_timestep's clean_state and mc_level_post_timestep does nothing. */
 typedef struct mc_level mc_level_t;
 enum { MC_BC_PERIODIC = 0, MC_BC_OUTFLOW = 1 };
 enum { MC_MODE_PLAIN = 0, MC_MODE_FUSED = 1 };
@@ -213,10 +217,19 @@ int mc_level_post_timestep(mc_level_t *L);      /* Source/Castro.cpp:343-369 */
 /* Amr::coarseTimeStep: computeInitialDt/computeNewDt, advance,
    post_timestep.  Returns the dt taken (negative on error). */
 double mc_level_coarse_step(mc_level_t *L, double stop_time);
+/* nsteps coarse steps; in fused mode dt stays on the device between steps and
+   the step may be replayed from a captured hipGraph (use_graph != 0). */
+int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph);
 int mc_level_steps(const mc_level_t *L);
 double mc_level_time(const mc_level_t *L);
 double mc_level_dt(const mc_level_t *L);
 int mc_level_sync(mc_level_t *L);
+/* Fused-mode tiling: stage 0..7 = est, prim, flat, trace, flux1, trans1,
+   final, update; a brick of bx*by*bz threads (multiple of 64, <= 256). */
+int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
+int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
+/* Launch one fused stage on the current device state (for timing). */
+int mc_level_run_stage(mc_level_t *L, int stage);
 
 #ifdef __cplusplus
 }

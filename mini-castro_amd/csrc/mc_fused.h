@@ -1,4 +1,5 @@
-// Batched fused CTU step over a whole box list (one launch per stage).
+// Batched fused CTU step: every box of a level, or every member of an
+// ensemble of independent instances, in one launch per stage.
 #pragma once
 
 #include <vector>
@@ -9,13 +10,31 @@ namespace mc {
 
 struct FusedLevel;
 
+struct FusedInstanceSpec {
+    int n[3];
+    double dx[3];
+    int bc;  // 0 periodic, 1 outflow
+};
+
+enum { FST_EST = 0, FST_PRIM, FST_FLAT, FST_TRACE, FST_FLUX1, FST_TRANS1, FST_FINAL, FST_UPDATE, FST_N };
+
+FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec);
+// single level: the box list only describes the state arena for import/export
 FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int bc, const double *dx);
 void fused_destroy(FusedLevel *F);
-// Estimate dt (min over cells, before the CFL factor) of the box-arena state S into *dt_dev.
-int fused_est_dt(FusedLevel *F, const double *S_arena, double *dt_dev, hipStream_t s);
-// One Castro::advance: S_new = clean(clean(S_old) + dt * hydro_source) on every box.
-int fused_advance(FusedLevel *F, const double *S_old_arena, double *S_new_arena, double dt, hipStream_t s);
-// Castro::post_timestep's clean_state on the box arena.
-int fused_clean(FusedLevel *F, double *S_arena, hipStream_t s);
+int fused_import(FusedLevel *F, const double *arena, hipStream_t s);
+int fused_export(FusedLevel *F, double *arena, hipStream_t s);
+double *fused_state(FusedLevel *F, int m);  // instance m state (20 comps, cell layout)
+long fused_ncell(FusedLevel *F, int m);
+int fused_ninst(FusedLevel *F);
+int fused_estimate(FusedLevel *F, hipStream_t s);
+int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool use_graph);
+int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s);
+int fused_set_clock(FusedLevel *F, int m, double dt, double time, int steps, hipStream_t s);
+int fused_read_clock(FusedLevel *F, int m, double *dt, double *time, int *steps, hipStream_t s);
+int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s);
+void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz);
+void fused_get_tiling(FusedLevel *F, int stage, int *bxyz);
+int fused_run_stage(FusedLevel *F, int stage, hipStream_t s);
 
 } // namespace mc

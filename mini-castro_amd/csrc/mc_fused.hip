@@ -1,21 +1,1017 @@
-// Batched fused CTU path (placeholder until the fused kernels land).
+// Batched fused CTU step for gfx950.
+//
+// All boxes of a level -- and all members of an ensemble of independent
+// Sedov instances -- advance together, one launch per stage:
+//
+//   K_prim    cells   clean_state(S_old), clean_state(Sborder), ctoprim
+//   K_flat    cells+1 shock flattening coefficient, node-centred div(u)
+//   K_trace   cells+1 PPM edge states in x, y, z (blockIdx.y = direction)
+//   K_flux1   faces   CGF fluxes of the PPM states           F^d
+//   K_trans1  faces   one-direction transverse update + flux  F^{t|d}
+//   K_final   faces   two-direction transverse update + final flux,
+//                     artificial viscosity, species flux normalisation
+//   K_update  cells   flux divergence and p dV source, S_new = S_old + dt
+//                     src, clean_state twice, estdt min-reduction
+//   K_dt      1 block the CFL / change_max / init_shrink time-step control
+//
+// Why the result equals the per-box path bit for bit: every stage applies the
+// same device functions (mc_cell.h, mc_hydro.h, mc_eos.h) to the same
+// inputs.  The reference's Sborder ghost cells hold clean(S_old) of the
+// source cell they are copied from, so every per-cell quantity (cleaned
+// state, primitives, sound speed) is a function of the source cell alone: it
+// is computed once per domain cell here and read through the boundary map
+// (periodic wrap or first-order-extrapolation clamp), where the per-box path
+// recomputes it for every ghost cell of every box.  Face quantities are
+// computed once per face instead of once per box that touches the face.
+// Quantities the reference evaluates on ghost cells of the domain (traces
+// and fluxes up to one cell outside) are evaluated on a one-cell ring
+// around each instance ("extended" arrays).
+//
+// Layout: per instance, structure-of-arrays; a cell array with C components
+// holds component c of cell (i,j,k) at C*off + c*ncell + i + n0*(j + n1*k);
+// extended arrays cover [-1, n] in each direction.  dt lives on the device
+// (one slot per instance), so a step can be captured in a hipGraph.
+#include <float.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mc_cell.h"
+#include "mc_common.h"
+#include "mc_eos.h"
 #include "mc_fused.h"
+#include "mc_hydro.h"
 #include "mc_internal.h"
 
 namespace mc {
 
-struct FusedLevel {
-    int dummy;
+// primitive components of the fused path
+enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_X, FQ_C = FQ_X + NSPEC, FQ_G, NFQ };
+// traced / corrected interface states (velocities in absolute order)
+enum { FS_RHO = 0, FS_U, FS_V, FS_W, FS_P, FS_RE, FS_X, NFS = FS_X + NSPEC };
+// transverse-stage fluxes: conserved URHO..UEDEN, species, Godunov p and normal u
+enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_X, FF_GP = FF_X + NSPEC, FF_GU, NFF };
+// final fluxes: NVAR conserved components, Godunov p and normal u
+enum { FN_GP = NVAR, FN_GU, NFN };
+
+struct FInst {
+    int n[3];
+    int periodic;
+    double dx[3];
+    long offD, offE;  // cell offsets in the cell / extended index spaces
+    long nD, nE;      // cell counts
+    int ext[3];       // n + 2
 };
 
-FusedLevel *fused_create(const std::vector<Box> &, const int *, int, const double *)
+// Brick tiling of a launch: blocks of bx*by*bz threads, one brick each.
+struct Plan {
+    const FInst *inst;
+    const int *blk0;  // first block of each instance, ninst+1 entries
+    int ninst;
+    int bx, by, bz;
+    int shift;        // first index: -shift
+    int add;          // extent: n + add (cells [0,n): 0,0; faces [0,n]: 0,1; extended [-1,n]: 1,2)
+};
+
+struct Ctx {
+    const FInst *inst;
+    double *U, *U1, *Q, *FL, *DV;
+    double *SM[3], *SP[3], *FD[3], *FT[6], *FN[3];
+    double *dt_cur, *dt_est, *time;
+    int *steps;
+    EosTable tab;
+};
+
+__device__ __forceinline__ int find_inst(const Plan &p)
 {
-    set_error("fused mode is not built yet");
-    return nullptr;
+    int lo = 0, hi = p.ninst - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
-void fused_destroy(FusedLevel *F) { delete F; }
-int fused_est_dt(FusedLevel *, const double *, double *, hipStream_t) { return -1; }
-int fused_advance(FusedLevel *, const double *, double *, double, hipStream_t) { return -1; }
-int fused_clean(FusedLevel *, double *, hipStream_t) { return -1; }
+
+// Cell of this thread in the plan's range; false if outside.
+__device__ __forceinline__ bool plan_cell(const Plan &p, int &m, int &i, int &j, int &k)
+{
+    m = find_inst(p);
+    const FInst &I = p.inst[m];
+    const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.n[2] + p.add;
+    const int nb0 = (e0 + p.bx - 1) / p.bx, nb1 = (e1 + p.by - 1) / p.by;
+    const int b = blockIdx.x - p.blk0[m];
+    const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
+    const int t = threadIdx.x;
+    const int ti = t % p.bx, tj = (t / p.bx) % p.by, tk = t / (p.bx * p.by);
+    const int li = bi * p.bx + ti, lj = bj * p.by + tj, lk = bk * p.bz + tk;
+    i = li - p.shift;
+    j = lj - p.shift;
+    k = lk - p.shift;
+    return li < e0 && lj < e1 && lk < e2;
+}
+
+__device__ __forceinline__ int bmap(int i, int n, int periodic)
+{
+    return periodic ? ((i % n) + n) % n : min(max(i, 0), n - 1);
+}
+
+// index of a (boundary-mapped) cell in the cell space of instance I
+__device__ __forceinline__ long cidx(const FInst &I, int i, int j, int k)
+{
+    return (long)bmap(i, I.n[0], I.periodic) +
+           (long)I.n[0] * (bmap(j, I.n[1], I.periodic) + (long)I.n[1] * bmap(k, I.n[2], I.periodic));
+}
+// index of an extended cell / face (i,j,k in [-1, n])
+__device__ __forceinline__ long eidx(const FInst &I, int i, int j, int k)
+{
+    return (long)(i + 1) + (long)I.ext[0] * ((j + 1) + (long)I.ext[1] * (k + 1));
+}
+// component c of a cell array with C components
+__device__ __forceinline__ double &CA(double *a, const FInst &I, int C, int c, long idx)
+{
+    return a[(long)C * I.offD + (long)c * I.nD + idx];
+}
+__device__ __forceinline__ double &EA(double *a, const FInst &I, int C, int c, long idx)
+{
+    return a[(long)C * I.offE + (long)c * I.nE + idx];
+}
+
+// ------------------------------------------------------------------ stages
+// estdt on the current state (Castro::estTimeStep before the first step)
+__global__ __launch_bounds__(256) void K_est(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    double v = DBL_MAX;
+    const bool ok = plan_cell(p, m, i, j, k);
+    const FInst &I = p.inst[m];
+    if (ok) {
+        const long c = cidx(I, i, j, k);
+        double u[NVAR];
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
+        v = estdt_cell(u, I.dx[0], I.dx[1], I.dx[2], x.tab);
+    }
+    v = block_min<256>(v);
+    if (threadIdx.x == 0) atomic_min_d(&x.dt_est[m], v);
+}
+
+// S_old clean (Castro_advance.cpp:51), Sborder clean (:66), ctoprim
+__global__ __launch_bounds__(256) void K_prim(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    if (!plan_cell(p, m, i, j, k)) return;
+    const FInst &I = p.inst[m];
+    const long c = cidx(I, i, j, k);
+    double u[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
+    clean_cell(u, x.tab);
+    clean_cell(u, x.tab);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) CA(x.U1, I, NVAR, n, c) = u[n];
+    Prim q;
+    ctoprim_cell(u, x.tab, q);
+    CA(x.Q, I, NFQ, FQ_RHO, c) = q.rho;
+    CA(x.Q, I, NFQ, FQ_U, c) = q.u;
+    CA(x.Q, I, NFQ, FQ_V, c) = q.v;
+    CA(x.Q, I, NFQ, FQ_W, c) = q.w;
+    CA(x.Q, I, NFQ, FQ_P, c) = q.p;
+    CA(x.Q, I, NFQ, FQ_RE, c) = q.rhoe;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) CA(x.Q, I, NFQ, FQ_X + n, c) = q.x[n];
+    CA(x.Q, I, NFQ, FQ_C, c) = q.cs;
+    CA(x.Q, I, NFQ, FQ_G, c) = q.gamc;
+}
+
+__device__ __forceinline__ double qv(const Ctx &x, const FInst &I, int comp, int i, int j, int k)
+{
+    return CA(x.Q, I, NFQ, comp, cidx(I, i, j, k));
+}
+
+// uflatten (Source/ppm.F90:11-128) and divu (Source/Castro_hydro_nd.F90:11-69)
+__global__ __launch_bounds__(256) void K_flat(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    if (!plan_cell(p, m, i, j, k)) return;
+    const FInst &I = p.inst[m];
+    const long e = eidx(I, i, j, k);
+    double flatn = 1.0;
+    for (int d = 0; d < 3; ++d) {
+        const int ei = (d == 0), ej = (d == 1), ek = (d == 2);
+        double p7[7], u7[7];
+#pragma unroll
+        for (int s = -3; s <= 3; ++s) {
+            const long c = cidx(I, i + s * ei, j + s * ej, k + s * ek);
+            p7[s + 3] = CA(x.Q, I, NFQ, FQ_P, c);
+            u7[s + 3] = CA(x.Q, I, NFQ, FQ_U + d, c);
+        }
+        flatn = fmin(flatn, flatten_dir(p7, u7));
+    }
+    EA(x.FL, I, 1, 0, e) = flatn;
+    if (i >= 0 && j >= 0 && k >= 0) {
+        const double dxinv = 1.0 / I.dx[0], dyinv = 1.0 / I.dx[1], dzinv = 1.0 / I.dx[2];
+#define QQ(c, a, b, cc) qv(x, I, c, a, b, cc)
+        const double ux = FOURTH *
+                          (+QQ(FQ_U, i, j, k) - QQ(FQ_U, i - 1, j, k) + QQ(FQ_U, i, j, k - 1) -
+                           QQ(FQ_U, i - 1, j, k - 1) + QQ(FQ_U, i, j - 1, k) - QQ(FQ_U, i - 1, j - 1, k) +
+                           QQ(FQ_U, i, j - 1, k - 1) - QQ(FQ_U, i - 1, j - 1, k - 1)) *
+                          dxinv;
+        const double vy = FOURTH *
+                          (+QQ(FQ_V, i, j, k) - QQ(FQ_V, i, j - 1, k) + QQ(FQ_V, i, j, k - 1) -
+                           QQ(FQ_V, i, j - 1, k - 1) + QQ(FQ_V, i - 1, j, k) - QQ(FQ_V, i - 1, j - 1, k) +
+                           QQ(FQ_V, i - 1, j, k - 1) - QQ(FQ_V, i - 1, j - 1, k - 1)) *
+                          dyinv;
+        const double wz = FOURTH *
+                          (+QQ(FQ_W, i, j, k) - QQ(FQ_W, i, j, k - 1) + QQ(FQ_W, i, j - 1, k) -
+                           QQ(FQ_W, i, j - 1, k - 1) + QQ(FQ_W, i - 1, j, k) - QQ(FQ_W, i - 1, j, k - 1) +
+                           QQ(FQ_W, i - 1, j - 1, k) - QQ(FQ_W, i - 1, j - 1, k - 1)) *
+                          dzinv;
+#undef QQ
+        EA(x.DV, I, 1, 0, e) = ux + vy + wz;
+    }
+}
+
+// PPM reconstruction + characteristic tracing (Source/ppm.F90:309-743)
+__global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    if (!plan_cell(p, m, i, j, k)) return;
+    const int d = blockIdx.y;
+    const FInst &I = p.inst[m];
+    const int di = (d == 0), dj = (d == 1), dk = (d == 2);
+    const int pos = d == 0 ? i : (d == 1 ? j : k);
+    const bool do_p = pos >= 0;
+    const bool do_m = pos <= I.n[d] - 1;
+    const int cun = FQ_U + d, cut = d == 0 ? FQ_V : (d == 1 ? FQ_W : FQ_U), cutt = d == 0 ? FQ_W : (d == 1 ? FQ_U : FQ_V);
+    const long c0 = cidx(I, i, j, k);
+    const double cc = CA(x.Q, I, NFQ, FQ_C, c0);
+    const double un = CA(x.Q, I, NFQ, cun, c0);
+    const double flatn = EA(x.FL, I, 1, 0, eidx(I, i, j, k));
+    const double dtdx = x.dt_cur[m] / I.dx[d];
+    long cs[5];
+#pragma unroll
+    for (int s = -2; s <= 2; ++s) cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
+
+    double Ip[3][4], Im[3][4], Ip_ut = 0, Im_ut = 0, Ip_utt = 0, Im_utt = 0;
+    const int vars[6] = {FQ_RHO, cun, FQ_P, FQ_RE, cut, cutt};
+    for (int vv = 0; vv < 6; ++vv) {
+        double s[5], sm, sp;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, vars[vv], cs[q]);
+        ppm_reconstruct(s, flatn, sm, sp);
+        const double s6 = ppm_s6(sm, sp, s[2]);
+        if (vv < 4) {
+            ppm_int_wave(sm, sp, s6, un - cc, dtdx, Ip[0][vv], Im[0][vv]);
+            ppm_int_wave(sm, sp, s6, un, dtdx, Ip[1][vv], Im[1][vv]);
+            ppm_int_wave(sm, sp, s6, un + cc, dtdx, Ip[2][vv], Im[2][vv]);
+        } else if (vv == 4) {
+            ppm_int_wave(sm, sp, s6, un, dtdx, Ip_ut, Im_ut);
+        } else {
+            ppm_int_wave(sm, sp, s6, un, dtdx, Ip_utt, Im_utt);
+        }
+    }
+    double gam_m, gam_p;
+    {
+        double s[5], sm, sp, a, b;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, FQ_G, cs[q]);
+        ppm_reconstruct(s, flatn, sm, sp);
+        const double s6 = ppm_s6(sm, sp, s[2]);
+        ppm_int_wave(sm, sp, s6, un - cc, dtdx, a, gam_m);
+        ppm_int_wave(sm, sp, s6, un + cc, dtdx, gam_p, b);
+        (void)a;
+        (void)b;
+    }
+    const long ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
+    double *SP = x.SP[d], *SM = x.SM[d];
+    for (int ns = 0; ns < NSPEC; ++ns) {
+        double s[5], sm, sp, ipv, imv;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, FQ_X + ns, cs[q]);
+        ppm_reconstruct(s, flatn, sm, sp);
+        const double s6 = ppm_s6(sm, sp, s[2]);
+        ppm_int_wave(sm, sp, s6, un, dtdx, ipv, imv);
+        if (do_p) EA(SP, I, NFS, FS_X + ns, ep) = imv;
+        if (do_m) EA(SM, I, NFS, FS_X + ns, em) = ipv;
+    }
+    const int sun = FS_U + d, sut = d == 0 ? FS_V : (d == 1 ? FS_W : FS_U), sutt = d == 0 ? FS_W : (d == 1 ? FS_U : FS_V);
+    if (do_p) {
+        double r, u_, e, pp;
+        ppm_project<true>(Im, gam_m, un, cc, r, u_, e, pp);
+        EA(SP, I, NFS, FS_RHO, ep) = r;
+        EA(SP, I, NFS, sun, ep) = u_;
+        EA(SP, I, NFS, FS_RE, ep) = e;
+        EA(SP, I, NFS, FS_P, ep) = pp;
+        EA(SP, I, NFS, sut, ep) = Im_ut;
+        EA(SP, I, NFS, sutt, ep) = Im_utt;
+    }
+    if (do_m) {
+        double r, u_, e, pp;
+        ppm_project<false>(Ip, gam_p, un, cc, r, u_, e, pp);
+        EA(SM, I, NFS, FS_RHO, em) = r;
+        EA(SM, I, NFS, sun, em) = u_;
+        EA(SM, I, NFS, FS_RE, em) = e;
+        EA(SM, I, NFS, FS_P, em) = pp;
+        EA(SM, I, NFS, sut, em) = Ip_ut;
+        EA(SM, I, NFS, sutt, em) = Ip_utt;
+    }
+}
+
+__device__ __forceinline__ void load_fs(double *S, const FInst &I, long e, int d, Face &f)
+{
+    const int un = FS_U + d, ut = d == 0 ? FS_V : FS_U, utt = d == 2 ? FS_V : FS_W;
+    f.rho = EA(S, I, NFS, FS_RHO, e);
+    f.un = EA(S, I, NFS, un, e);
+    f.ut = EA(S, I, NFS, ut, e);
+    f.utt = EA(S, I, NFS, utt, e);
+    f.p = EA(S, I, NFS, FS_P, e);
+    f.rhoe = EA(S, I, NFS, FS_RE, e);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) f.x[n] = EA(S, I, NFS, FS_X + n, e);
+}
+
+__device__ __forceinline__ void qstate_to_face(const QState &o, int d, Face &f)
+{
+    f.rho = o.rho;
+    f.un = d == 0 ? o.u : (d == 1 ? o.v : o.w);
+    f.ut = d == 0 ? o.v : o.u;
+    f.utt = d == 2 ? o.v : o.w;
+    f.p = o.p;
+    f.rhoe = o.rhoe;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) f.x[n] = o.x[n];
+}
+
+// Fluxes in the transverse-stage layout; momenta back in absolute order.
+__device__ __forceinline__ void store_tflux(double *F, const FInst &I, long e, int d, const RiemannOut &o)
+{
+    const int mn = FF_MX + d, mt = d == 0 ? FF_MY : FF_MX, mtt = d == 2 ? FF_MY : FF_MZ;
+    EA(F, I, NFF, FF_RHO, e) = o.f_rho;
+    EA(F, I, NFF, mn, e) = o.f_mn;
+    EA(F, I, NFF, mt, e) = o.f_mt;
+    EA(F, I, NFF, mtt, e) = o.f_mtt;
+    EA(F, I, NFF, FF_E, e) = o.f_E;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) EA(F, I, NFF, FF_X + n, e) = o.f_rho * o.gx[n];
+    EA(F, I, NFF, FF_GP, e) = o.g_p;
+    EA(F, I, NFF, FF_GU, e) = o.g_un;
+}
+
+// Riemann problem at face e (cell index of its high side (i,j,k)) in direction d
+__device__ __forceinline__ void face_riemann(const Ctx &x, const FInst &I, int d, int i, int j, int k, const Face &L,
+                                             const Face &R, RiemannOut &o)
+{
+    const long cr = cidx(I, i, j, k);
+    const long cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
+    riemann_cgf(L, R, CA(x.Q, I, NFQ, FQ_C, cl), CA(x.Q, I, NFQ, FQ_C, cr), CA(x.Q, I, NFQ, FQ_G, cl),
+                CA(x.Q, I, NFQ, FQ_G, cr), o);
+}
+
+// Source/riemann.F90:10-331 on the PPM states: F^d on d-faces
+// (d-face index 0..n_d, transverse cells -1..n)
+__global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    if (!plan_cell(p, m, i, j, k)) return;
+    const int d = blockIdx.y;
+    const FInst &I = p.inst[m];
+    const int pos = d == 0 ? i : (d == 1 ? j : k);
+    if (pos < 0) return;
+    const long e = eidx(I, i, j, k);
+    Face L, R;
+    load_fs(x.SM[d], I, e, d, L);
+    load_fs(x.SP[d], I, e, d, R);
+    RiemannOut o;
+    face_riemann(x, I, d, i, j, k, L, R, o);
+    store_tflux(x.FD[d], I, e, d, o);
+}
+
+__device__ __forceinline__ void load_qin_fs(double *S, const FInst &I, long e, QIn &q)
+{
+    q.rho = EA(S, I, NFS, FS_RHO, e);
+    q.u = EA(S, I, NFS, FS_U, e);
+    q.v = EA(S, I, NFS, FS_V, e);
+    q.w = EA(S, I, NFS, FS_W, e);
+    q.p = EA(S, I, NFS, FS_P, e);
+    q.rhoe = EA(S, I, NFS, FS_RE, e);
+    q.game = 0.0;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) q.x[n] = EA(S, I, NFS, FS_X + n, e);
+}
+
+// flux components a transverse update reads, in conserved-component slots
+__device__ __forceinline__ void load_tflux(double *F, const FInst &I, long e, double *v, double &gp, double &gu)
+{
+#pragma unroll
+    for (int n = 0; n <= UEDEN; ++n) v[n] = EA(F, I, NFF, FF_RHO + n, e);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) v[UFS + n] = EA(F, I, NFF, FF_X + n, e);
+    gp = EA(F, I, NFF, FF_GP, e);
+    gu = EA(F, I, NFF, FF_GU, e);
+}
+
+__constant__ int c_pair_t[6] = {1, 2, 0, 2, 0, 1};   // state (face) direction
+__constant__ int c_pair_d[6] = {0, 0, 1, 1, 2, 2};   // flux direction
+
+// trans1 (Source/trans.F90:14-243) of the t-face states by the d-flux, then
+// the flux F^{t|d} of the corrected states on the t-faces of tbx[t][d]
+__global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    if (!plan_cell(p, m, i, j, k)) return;
+    const int combo = blockIdx.y;
+    const int t = c_pair_t[combo], d = c_pair_d[combo];
+    const FInst &I = p.inst[m];
+    const int c[3] = {i, j, k};
+    // range: t-face 0..n_t, d-cells 0..n_d-1, the third direction -1..n
+    if (c[t] < 0 || c[d] < 0 || c[d] > I.n[d] - 1) return;
+    const long e = eidx(I, i, j, k);
+    const int et[3] = {t == 0, t == 1, t == 2}, ed[3] = {d == 0, d == 1, d == 2};
+    const double cdtdx = x.dt_cur[m] / I.dx[d] / 3.0;
+    Face side[2];
+    for (int s = 0; s < 2; ++s) {
+        const int sh = s - 1;  // -1: minus state, 0: plus state
+        const int il = i + sh * et[0], jl = j + sh * et[1], kl = k + sh * et[2];
+        QIn lq;
+        load_qin_fs(s == 0 ? x.SM[t] : x.SP[t], I, e, lq);
+        double fr[NVAR], fl[NVAR], pgp, ugp, pgm, ugm;
+        load_tflux(x.FD[d], I, eidx(I, il + ed[0], jl + ed[1], kl + ed[2]), fr, pgp, ugp);
+        load_tflux(x.FD[d], I, eidx(I, il, jl, kl), fl, pgm, ugm);
+        const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, il, jl, kl));
+        const QState o = trans1_state(lq, fr, fl, pgp, pgm, ugp, ugm, gamc, cdtdx);
+        qstate_to_face(o, t, side[s]);
+    }
+    RiemannOut o;
+    face_riemann(x, I, t, i, j, k, side[0], side[1], o);
+    store_tflux(x.FT[combo], I, e, t, o);
+}
+
+// index of the F^{a|b} array
+__device__ __forceinline__ int pair_index(int a, int b)
+{
+    // combos: (1,0) (2,0) (0,1) (2,1) (0,2) (1,2)
+    return 2 * b + ((a == 2) || (b == 2 && a == 1) ? 1 : 0);
+}
+
+// trans2 (Source/trans.F90:250-505) + final flux + apply_av
+// (Source/Castro_hydro_nd.F90:174-245) + normalize_species_fluxes (:248-297)
+// on the d-faces of the valid region
+__global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    if (!plan_cell(p, m, i, j, k)) return;
+    const int d = blockIdx.y;
+    const FInst &I = p.inst[m];
+    const int c[3] = {i, j, k};
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] < 0) return;
+        if (a != d && c[a] > I.n[a] - 1) return;
+    }
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int ed[3] = {d == 0, d == 1, d == 2}, e1[3] = {t1 == 0, t1 == 1, t1 == 2}, e2[3] = {t2 == 0, t2 == 1, t2 == 2};
+    const double dt = x.dt_cur[m];
+    const double h2 = 0.5 * dt / I.dx[t1], h3 = 0.5 * dt / I.dx[t2];
+    const long e = eidx(I, i, j, k);
+    double *F2 = x.FT[pair_index(t1, t2)], *F3 = x.FT[pair_index(t2, t1)];
+    Face side[2];
+    for (int s = 0; s < 2; ++s) {
+        const int sh = s - 1;
+        const int il = i + sh * ed[0], jl = j + sh * ed[1], kl = k + sh * ed[2];
+        QIn lq;
+        load_qin_fs(s == 0 ? x.SM[d] : x.SP[d], I, e, lq);
+        double f2r[NVAR], f2l[NVAR], f3r[NVAR], f3l[NVAR], pg2p, ug2p, pg2m, ug2m, pg3p, ug3p, pg3m, ug3m;
+        load_tflux(F2, I, eidx(I, il + e1[0], jl + e1[1], kl + e1[2]), f2r, pg2p, ug2p);
+        load_tflux(F2, I, eidx(I, il, jl, kl), f2l, pg2m, ug2m);
+        load_tflux(F3, I, eidx(I, il + e2[0], jl + e2[1], kl + e2[2]), f3r, pg3p, ug3p);
+        load_tflux(F3, I, eidx(I, il, jl, kl), f3l, pg3m, ug3m);
+        const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, il, jl, kl));
+        const QState o = trans2_state(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m, gamc,
+                                      h2, h3);
+        qstate_to_face(o, d, side[s]);
+    }
+    RiemannOut o;
+    face_riemann(x, I, d, i, j, k, side[0], side[1], o);
+    double f[NVAR];
+    const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
+    f[URHO] = o.f_rho;
+    f[mn] = o.f_mn;
+    f[mt] = o.f_mt;
+    f[mtt] = o.f_mtt;
+    f[UEDEN] = o.f_E;
+    f[UEINT] = o.f_e;
+    f[UTEMP] = 0.0;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) f[UFS + n] = o.f_rho * o.gx[n];
+
+    // artificial viscosity with the reference's corner-node pattern
+    double div1;
+#define DVN(a, b, cc) EA(x.DV, I, 1, 0, eidx(I, a, b, cc))
+    if (d == 0) div1 = FOURTH * (DVN(i, j, k) + DVN(i, j + 1, k) + DVN(i, j, k + 1) + DVN(i, j + 1, k));
+    else if (d == 1) div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j, k + 1) + DVN(i + 1, j, k));
+    else div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j + 1, k) + DVN(i + 1, j + 1, k));
+#undef DVN
+    div1 = 0.1 * fmin(0.0, div1);
+    const long cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
+    const double dxd = I.dx[d];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        if (n == UTEMP) continue;
+        const double d1 = div1 * (CA(x.U1, I, NVAR, n, cr) - CA(x.U1, I, NVAR, n, cl));
+        f[n] = f[n] + dxd * d1;
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int n = UFS; n < UFS + NSPEC; ++n) sum = sum + f[n];
+    const double fac = (sum != 0.0) ? f[URHO] / sum : 1.0;
+#pragma unroll
+    for (int n = UFS; n < UFS + NSPEC; ++n) f[n] = f[n] * fac;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) EA(x.FN[d], I, NFN, n, e) = f[n];
+    EA(x.FN[d], I, NFN, FN_GP, e) = o.g_p;
+    EA(x.FN[d], I, NFN, FN_GU, e) = o.g_un;
+}
+
+// fill_hydro_source (Source/Castro_hydro_nd.F90:344-441), Saxpy and the two
+// clean_state calls of Castro_advance.cpp:74-78 / Castro.cpp:369, then the
+// next step's estdt (Source/timestep.F90:12-72) min-reduced per instance.
+__global__ __launch_bounds__(256) void K_update(Plan p, Ctx x)
+{
+    int m, i, j, k;
+    double v = DBL_MAX;
+    const bool ok = plan_cell(p, m, i, j, k);
+    const FInst &I = p.inst[m];
+    if (ok) {
+        const long c = cidx(I, i, j, k);
+        const double dx0 = I.dx[0], dx1 = I.dx[1], dx2 = I.dx[2];
+        const double a0 = dx1 * dx2, a1 = dx0 * dx2, a2 = dx0 * dx1;
+        const double volinv = 1.0 / (dx0 * dx1 * dx2);
+        const long e = eidx(I, i, j, k);
+        const long ex = eidx(I, i + 1, j, k), ey = eidx(I, i, j + 1, k), ez = eidx(I, i, j, k + 1);
+        double *F0 = x.FN[0], *F1 = x.FN[1], *F2 = x.FN[2];
+        const double dt = x.dt_cur[m];
+        double u[NVAR];
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
+        clean_cell(u, x.tab); /* S_old clean, Castro_advance.cpp:51 */
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) {
+            double s = 0.0 + (EA(F0, I, NFN, n, e) * a0 - EA(F0, I, NFN, n, ex) * a0 + EA(F1, I, NFN, n, e) * a1 -
+                              EA(F1, I, NFN, n, ey) * a1 + EA(F2, I, NFN, n, e) * a2 - EA(F2, I, NFN, n, ez) * a2) *
+                                 volinv;
+            if (n == UEINT) {
+                const double pdivu =
+                    HALF * (EA(F0, I, NFN, FN_GP, ex) + EA(F0, I, NFN, FN_GP, e)) *
+                        (EA(F0, I, NFN, FN_GU, ex) - EA(F0, I, NFN, FN_GU, e)) / dx0 +
+                    HALF * (EA(F1, I, NFN, FN_GP, ey) + EA(F1, I, NFN, FN_GP, e)) *
+                        (EA(F1, I, NFN, FN_GU, ey) - EA(F1, I, NFN, FN_GU, e)) / dx1 +
+                    HALF * (EA(F2, I, NFN, FN_GP, ez) + EA(F2, I, NFN, FN_GP, e)) *
+                        (EA(F2, I, NFN, FN_GU, ez) - EA(F2, I, NFN, FN_GU, e)) / dx2;
+                s = s - pdivu;
+            }
+            u[n] = u[n] + dt * s;
+        }
+        clean_cell(u, x.tab); /* Castro_advance.cpp:78 */
+        clean_cell(u, x.tab); /* post_timestep, Castro.cpp:369 */
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) CA(x.U, I, NVAR, n, c) = u[n];
+        v = estdt_cell(u, dx0, dx1, dx2, x.tab);
+    }
+    v = block_min<256>(v);
+    if (threadIdx.x == 0) atomic_min_d(&x.dt_est[m], v);
+}
+
+// Time-step control of Castro::computeInitialDt / computeNewDt
+// (Source/Castro.cpp:162-173, 175-216, 218-341) for every instance.
+__global__ void K_dt(Ctx x, int ninst, double stop_time)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= ninst) return;
+    const double est = x.dt_est[m] * cfl;
+    double dt_0;
+    if (x.steps[m] > 0) {
+        double dt_min = fmin(est, 1.1e0 * x.dt_cur[m]);
+        dt_0 = fmin(1.0e+100, dt_min);
+    } else {
+        dt_0 = fmin(1.0e+100, 0.01 * est);
+    }
+    const double eps = 0.001 * dt_0;
+    if (stop_time >= 0.0)
+        if ((x.time[m] + dt_0) > (stop_time - eps)) dt_0 = stop_time - x.time[m];
+    x.dt_cur[m] = dt_0;
+    x.dt_est[m] = DBL_MAX;
+}
+
+__global__ void K_tick(Ctx x, int ninst)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= ninst) return;
+    x.time[m] += x.dt_cur[m];
+    x.steps[m] += 1;
+}
+
+__global__ void K_fill(double *p, long n, double v)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// box arena <-> instance-0 cell layout
+struct ArenaBox {
+    int lo[3], hi[3];
+    long off;
+};
+__global__ __launch_bounds__(256) void K_arena(const ArenaBox *boxes, int nbox, const long *bstart, double *arena,
+                                               double *U, long nD, int n0, int n1, int to_arena)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= bstart[nbox]) return;
+    int b = 0;
+    while (b + 1 < nbox && bstart[b + 1] <= t) ++b;
+    const ArenaBox B = boxes[b];
+    const long lt = t - bstart[b];
+    const int bx = B.hi[0] - B.lo[0] + 1, by = B.hi[1] - B.lo[1] + 1;
+    const long nb = lt < 0 ? 0 : (long)bx * by * (B.hi[2] - B.lo[2] + 1);
+    const int i = B.lo[0] + (int)(lt % bx), j = B.lo[1] + (int)((lt / bx) % by), k = B.lo[2] + (int)(lt / ((long)bx * by));
+    const long c = (long)i + (long)n0 * (j + (long)n1 * k);
+    for (int n = 0; n < NVAR; ++n) {
+        double *a = arena + B.off + nb * n + lt;
+        double *u = U + nD * n + c;
+        if (to_arena) *a = *u;
+        else *u = *a;
+    }
+}
+
+// ------------------------------------------------------------------ host side
+struct KernelTiling {
+    int bx, by, bz;
+};
+
+struct FusedLevel {
+    std::vector<FInst> hinst;
+    int ninst = 0;
+    FInst *inst = nullptr;
+    long totD = 0, totE = 0;
+    Ctx ctx;
+    double *arena_buf = nullptr;
+    // plans: one block table per (tiling, ghost) pair
+    struct PlanBuf {
+        KernelTiling t;
+        int shift, add;
+        int *blk0 = nullptr;
+        int nblocks = 0;
+    };
+    std::vector<PlanBuf> plans;
+    KernelTiling tiling[8];  // per stage: est prim flat trace flux1 trans1 final update
+    bool have_est = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double graph_stop = -2.0;
+    // single-level import/export
+    ArenaBox *abox = nullptr;
+    long *abstart = nullptr;
+    int nabox = 0;
+    long arena_n = 0;
+};
+
+enum { ST_EST = FST_EST, ST_PRIM = FST_PRIM, ST_FLAT = FST_FLAT, ST_TRACE = FST_TRACE, ST_FLUX1 = FST_FLUX1,
+       ST_TRANS1 = FST_TRANS1, ST_FINAL = FST_FINAL, ST_UPDATE = FST_UPDATE, NSTAGE = FST_N };
+// index range of each stage: cells [0,n), faces [0,n], or the ring [-1,n]
+static const int stage_shift[NSTAGE] = {0, 0, 1, 1, 1, 1, 0, 0};
+static const int stage_add[NSTAGE] = {0, 0, 2, 2, 2, 2, 1, 0};
+
+static Plan get_plan(FusedLevel *F, int stage)
+{
+    const KernelTiling t = F->tiling[stage];
+    const int sh = stage_shift[stage], add = stage_add[stage];
+    for (auto &pb : F->plans)
+        if (pb.t.bx == t.bx && pb.t.by == t.by && pb.t.bz == t.bz && pb.shift == sh && pb.add == add)
+            return Plan{F->inst, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+    FusedLevel::PlanBuf pb;
+    pb.t = t;
+    pb.shift = sh;
+    pb.add = add;
+    std::vector<int> b0(F->ninst + 1);
+    int acc = 0;
+    for (int m = 0; m < F->ninst; ++m) {
+        b0[m] = acc;
+        const FInst &I = F->hinst[m];
+        const int e0 = I.n[0] + add, e1 = I.n[1] + add, e2 = I.n[2] + add;
+        acc += ((e0 + t.bx - 1) / t.bx) * ((e1 + t.by - 1) / t.by) * ((e2 + t.bz - 1) / t.bz);
+    }
+    b0[F->ninst] = acc;
+    pb.nblocks = acc;
+    if (check_hip(hipMalloc(&pb.blk0, sizeof(int) * (F->ninst + 1)), "hipMalloc(plan)") ||
+        check_hip(hipMemcpy(pb.blk0, b0.data(), sizeof(int) * (F->ninst + 1), hipMemcpyHostToDevice), "plan h2d"))
+        return Plan{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
+    F->plans.push_back(pb);
+    return Plan{F->inst, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+}
+
+static int plan_blocks(FusedLevel *F, const Plan &p)
+{
+    for (auto &pb : F->plans)
+        if (pb.blk0 == p.blk0) return pb.nblocks;
+    return 0;
+}
+
+} // namespace mc
+
+using namespace mc;
+
+namespace mc {
+
+static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1)
+{
+    F->ctx.tab = eos_table();
+    Plan p = get_plan(F, stage);
+    const int nb = plan_blocks(F, p);
+    const int nt = p.bx * p.by * p.bz;
+    if (nb <= 0) return;
+    dim3 g(nb, ny), b(nt);
+    switch (stage) {
+    case ST_EST: hipLaunchKernelGGL(K_est, g, b, 0, s, p, F->ctx); break;
+    case ST_PRIM: hipLaunchKernelGGL(K_prim, g, b, 0, s, p, F->ctx); break;
+    case ST_FLAT: hipLaunchKernelGGL(K_flat, g, b, 0, s, p, F->ctx); break;
+    case ST_TRACE: hipLaunchKernelGGL(K_trace, g, b, 0, s, p, F->ctx); break;
+    case ST_FLUX1: hipLaunchKernelGGL(K_flux1, g, b, 0, s, p, F->ctx); break;
+    case ST_TRANS1: hipLaunchKernelGGL(K_trans1, g, b, 0, s, p, F->ctx); break;
+    case ST_FINAL: hipLaunchKernelGGL(K_final, g, b, 0, s, p, F->ctx); break;
+    case ST_UPDATE: hipLaunchKernelGGL(K_update, g, b, 0, s, p, F->ctx); break;
+    }
+}
+
+static void fill(double *p, long n, double v, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(K_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
+{
+    FusedLevel *F = new FusedLevel();
+    F->ninst = (int)spec.size();
+    long offD = 0, offE = 0;
+    for (const auto &sp : spec) {
+        FInst I;
+        for (int d = 0; d < 3; ++d) {
+            I.n[d] = sp.n[d];
+            I.dx[d] = sp.dx[d];
+            I.ext[d] = sp.n[d] + 2;
+        }
+        I.periodic = sp.bc == 0;
+        I.nD = (long)I.n[0] * I.n[1] * I.n[2];
+        I.nE = (long)I.ext[0] * I.ext[1] * I.ext[2];
+        I.offD = offD;
+        I.offE = offE;
+        offD += I.nD;
+        offE += I.nE;
+        F->hinst.push_back(I);
+    }
+    F->totD = offD;
+    F->totE = offE;
+    for (int s = 0; s < NSTAGE; ++s) F->tiling[s] = KernelTiling{64, 2, 2};
+    F->tiling[ST_EST] = F->tiling[ST_PRIM] = F->tiling[ST_UPDATE] = KernelTiling{64, 4, 1};
+    bool ok = !check_hip(hipMalloc(&F->inst, sizeof(FInst) * F->ninst), "hipMalloc(inst)") &&
+              !check_hip(hipMemcpy(F->inst, F->hinst.data(), sizeof(FInst) * F->ninst, hipMemcpyHostToDevice), "h2d");
+    auto A = [&](double *&p, long n) {
+        if (ok && check_hip(hipMalloc(&p, sizeof(double) * std::max(n, 1L)), "hipMalloc(fused)")) ok = false;
+    };
+    Ctx &x = F->ctx;
+    memset(&x, 0, sizeof(x));
+    x.inst = F->inst;
+    A(x.U, NVAR * offD);
+    A(x.U1, NVAR * offD);
+    A(x.Q, NFQ * offD);
+    A(x.FL, offE);
+    A(x.DV, offE);
+    for (int d = 0; d < 3; ++d) {
+        A(x.SM[d], NFS * offE);
+        A(x.SP[d], NFS * offE);
+        A(x.FD[d], NFF * offE);
+        A(x.FN[d], NFN * offE);
+    }
+    for (int c = 0; c < 6; ++c) A(x.FT[c], NFF * offE);
+    A(x.dt_cur, F->ninst);
+    A(x.dt_est, F->ninst);
+    A(x.time, F->ninst);
+    if (ok && check_hip(hipMalloc(&x.steps, sizeof(int) * F->ninst), "hipMalloc(steps)")) ok = false;
+    if (!ok) {
+        fused_destroy(F);
+        return nullptr;
+    }
+    x.tab = eos_table();
+    std::vector<int> z(F->ninst, 0);
+    (void)hipMemcpy(x.steps, z.data(), sizeof(int) * F->ninst, hipMemcpyHostToDevice);
+    fill(x.time, F->ninst, 0.0, nullptr);
+    fill(x.dt_cur, F->ninst, 0.0, nullptr);
+    fill(x.dt_est, F->ninst, DBL_MAX, nullptr);
+    (void)hipDeviceSynchronize();
+    return F;
+}
+
+FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int bc, const double *dx)
+{
+    FusedInstanceSpec sp;
+    for (int d = 0; d < 3; ++d) {
+        sp.n[d] = n_cell[d];
+        sp.dx[d] = dx[d];
+    }
+    sp.bc = bc;
+    FusedLevel *F = fused_create_set({sp});
+    if (!F) return nullptr;
+    // box arena description for import / export
+    std::vector<ArenaBox> ab;
+    std::vector<long> bs;
+    long off = 0, acc = 0;
+    for (const Box &b : boxes) {
+        ArenaBox a;
+        for (int d = 0; d < 3; ++d) {
+            a.lo[d] = b.lo[d];
+            a.hi[d] = b.hi[d];
+        }
+        a.off = off;
+        off += box_npts(b) * NVAR;
+        ab.push_back(a);
+        bs.push_back(acc);
+        acc += box_npts(b);
+    }
+    bs.push_back(acc);
+    F->nabox = (int)boxes.size();
+    F->arena_n = off;
+    if (check_hip(hipMalloc(&F->abox, sizeof(ArenaBox) * ab.size()), "hipMalloc(abox)") ||
+        check_hip(hipMalloc(&F->abstart, sizeof(long) * bs.size()), "hipMalloc(abstart)") ||
+        check_hip(hipMemcpy(F->abox, ab.data(), sizeof(ArenaBox) * ab.size(), hipMemcpyHostToDevice), "h2d") ||
+        check_hip(hipMemcpy(F->abstart, bs.data(), sizeof(long) * bs.size(), hipMemcpyHostToDevice), "h2d")) {
+        fused_destroy(F);
+        return nullptr;
+    }
+    return F;
+}
+
+void fused_destroy(FusedLevel *F)
+{
+    if (!F) return;
+    (void)hipDeviceSynchronize();
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    Ctx &x = F->ctx;
+    double *ps[] = {x.U, x.U1, x.Q, x.FL, x.DV, x.dt_cur, x.dt_est, x.time};
+    for (double *p : ps)
+        if (p) (void)hipFree(p);
+    for (int d = 0; d < 3; ++d) {
+        double *pd[] = {x.SM[d], x.SP[d], x.FD[d], x.FN[d]};
+        for (double *p : pd)
+            if (p) (void)hipFree(p);
+    }
+    for (int c = 0; c < 6; ++c)
+        if (x.FT[c]) (void)hipFree(x.FT[c]);
+    if (x.steps) (void)hipFree(x.steps);
+    for (auto &pb : F->plans)
+        if (pb.blk0) (void)hipFree(pb.blk0);
+    if (F->inst) (void)hipFree(F->inst);
+    if (F->abox) (void)hipFree(F->abox);
+    if (F->abstart) (void)hipFree(F->abstart);
+    delete F;
+}
+
+int fused_import(FusedLevel *F, const double *arena, hipStream_t s)
+{
+    const long n = F->hinst[0].nD;
+    hipLaunchKernelGGL(K_arena, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, F->abox, F->nabox, F->abstart,
+                       const_cast<double *>(arena), F->ctx.U, n, F->hinst[0].n[0], F->hinst[0].n[1], 0);
+    return check_hip(hipGetLastError(), "fused_import");
+}
+
+int fused_export(FusedLevel *F, double *arena, hipStream_t s)
+{
+    const long n = F->hinst[0].nD;
+    hipLaunchKernelGGL(K_arena, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, F->abox, F->nabox, F->abstart,
+                       arena, F->ctx.U, n, F->hinst[0].n[0], F->hinst[0].n[1], 1);
+    return check_hip(hipGetLastError(), "fused_export");
+}
+
+double *fused_state(FusedLevel *F, int m) { return F->ctx.U + (long)NVAR * F->hinst[m].offD; }
+long fused_ncell(FusedLevel *F, int m) { return F->hinst[m].nD; }
+int fused_ninst(FusedLevel *F) { return F->ninst; }
+
+// estdt of the current state of every instance into dt_est (no cleaning)
+int fused_estimate(FusedLevel *F, hipStream_t s)
+{
+    fill(F->ctx.dt_est, F->ninst, DBL_MAX, s);
+    launch_stage(F, ST_EST, s);
+    F->have_est = true;
+    return check_hip(hipGetLastError(), "fused_estimate");
+}
+
+// enqueue one coarse step of every instance
+static void enqueue_step(FusedLevel *F, double stop_time, hipStream_t s)
+{
+    F->ctx.tab = eos_table();
+    hipLaunchKernelGGL(K_dt, dim3(1), dim3(256), 0, s, F->ctx, F->ninst, stop_time);
+    launch_stage(F, ST_PRIM, s);
+    launch_stage(F, ST_FLAT, s);
+    launch_stage(F, ST_TRACE, s, 3);
+    launch_stage(F, ST_FLUX1, s, 3);
+    launch_stage(F, ST_TRANS1, s, 6);
+    launch_stage(F, ST_FINAL, s, 3);
+    launch_stage(F, ST_UPDATE, s);
+    hipLaunchKernelGGL(K_tick, dim3(1), dim3(256), 0, s, F->ctx, F->ninst);
+}
+
+int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool use_graph)
+{
+    if (!F->have_est && fused_estimate(F, s)) return -1;
+    if (!use_graph) {
+        for (int n = 0; n < nsteps; ++n) enqueue_step(F, stop_time, s);
+        return check_hip(hipGetLastError(), "fused_steps");
+    }
+    if (!F->gexec || F->graph_stop != stop_time) {
+        if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+        if (F->graph) (void)hipGraphDestroy(F->graph);
+        F->gexec = nullptr;
+        F->graph = nullptr;
+        // make sure every plan table exists before capture
+        for (int st = 0; st < NSTAGE; ++st) (void)get_plan(F, st);
+        if (check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) return -1;
+        enqueue_step(F, stop_time, s);
+        if (check_hip(hipStreamEndCapture(s, &F->graph), "end capture")) return -1;
+        if (check_hip(hipGraphInstantiate(&F->gexec, F->graph, nullptr, nullptr, 0), "graph instantiate")) return -1;
+        F->graph_stop = stop_time;
+    }
+    for (int n = 0; n < nsteps; ++n)
+        if (check_hip(hipGraphLaunch(F->gexec, s), "graph launch")) return -1;
+    return 0;
+}
+
+int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s)
+{
+    fill(F->ctx.dt_cur, F->ninst, dt, s);
+    fill(F->ctx.dt_est, F->ninst, DBL_MAX, s);
+    launch_stage(F, ST_PRIM, s);
+    launch_stage(F, ST_FLAT, s);
+    launch_stage(F, ST_TRACE, s, 3);
+    launch_stage(F, ST_FLUX1, s, 3);
+    launch_stage(F, ST_TRANS1, s, 6);
+    launch_stage(F, ST_FINAL, s, 3);
+    launch_stage(F, ST_UPDATE, s);
+    F->have_est = true;
+    return check_hip(hipGetLastError(), "fused_advance_dt");
+}
+
+__global__ void K_setclock(Ctx x, int m, double dt, double time, int steps)
+{
+    x.dt_cur[m] = dt;
+    x.time[m] = time;
+    x.steps[m] = steps;
+}
+
+int fused_set_clock(FusedLevel *F, int m, double dt, double time, int steps, hipStream_t s)
+{
+    hipLaunchKernelGGL(K_setclock, dim3(1), dim3(1), 0, s, F->ctx, m, dt, time, steps);
+    return check_hip(hipGetLastError(), "fused_set_clock");
+}
+
+int fused_read_clock(FusedLevel *F, int m, double *dt, double *time, int *steps, hipStream_t s)
+{
+    if (check_hip(hipMemcpyAsync(dt, F->ctx.dt_cur + m, sizeof(double), hipMemcpyDeviceToHost, s), "d2h") ||
+        check_hip(hipMemcpyAsync(time, F->ctx.time + m, sizeof(double), hipMemcpyDeviceToHost, s), "d2h") ||
+        check_hip(hipMemcpyAsync(steps, F->ctx.steps + m, sizeof(int), hipMemcpyDeviceToHost, s), "d2h"))
+        return -1;
+    return check_hip(hipStreamSynchronize(s), "clock sync");
+}
+
+int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s)
+{
+    if (check_hip(hipMemcpyAsync(est, F->ctx.dt_est + m, sizeof(double), hipMemcpyDeviceToHost, s), "d2h"))
+        return -1;
+    return check_hip(hipStreamSynchronize(s), "est sync");
+}
+
+void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz)
+{
+    if (stage < 0 || stage >= NSTAGE) return;
+    F->tiling[stage] = KernelTiling{bx, by, bz};
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    F->gexec = nullptr;
+    F->graph = nullptr;
+}
+
+void fused_get_tiling(FusedLevel *F, int stage, int *bxyz)
+{
+    bxyz[0] = F->tiling[stage].bx;
+    bxyz[1] = F->tiling[stage].by;
+    bxyz[2] = F->tiling[stage].bz;
+}
+
+int fused_run_stage(FusedLevel *F, int stage, hipStream_t s)
+{
+    static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    launch_stage(F, stage, s, ny[stage]);
+    return check_hip(hipGetLastError(), "fused_run_stage");
+}
 
 } // namespace mc

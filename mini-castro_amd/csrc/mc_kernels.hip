@@ -19,6 +19,7 @@
 #include "mc_common.h"
 #include "mc_eos.h"
 #include "mc_hydro.h"
+#include "mc_cell.h"
 #include "mc_internal.h"
 
 namespace mc {
@@ -111,39 +112,25 @@ static int eos_upload(const double *tab)
 __global__ __launch_bounds__(kBlock) void k_ctoprim(BoxIter it, Arr u, Arr q, Arr qaux, EosTable tab)
 {
     MC_CELL(it)
-    const double rho = u(i, j, k, URHO);
-    q(i, j, k, QRHO) = rho;
-    const double rhoinv = 1.0 / rho;
-    const double vx = u(i, j, k, UMX) * rhoinv, vy = u(i, j, k, UMY) * rhoinv, vz = u(i, j, k, UMZ) * rhoinv;
-    q(i, j, k, QU) = vx;
-    q(i, j, k, QV) = vy;
-    q(i, j, k, QW) = vz;
-    const double kineng = HALF * rho * (vx * vx + vy * vy + vz * vz);
-    const double E = u(i, j, k, UEDEN);
-    double e;
-    if ((E - kineng) / E > dual_energy_eta1) e = (E - kineng) * rhoinv;
-    else e = u(i, j, k, UEINT) * rhoinv;
-    double s1 = 0.0, s2 = 0.0;
+    double uc[NVAR];
 #pragma unroll
-    for (int n = 0; n < NSPEC; ++n) {
-        const double x = u(i, j, k, UFS + n) * rhoinv;
-        q(i, j, k, QFS + n) = x;
-        s1 = s1 + x * (1.0 / aion(n));
-    }
+    for (int n = 0; n < NVAR; ++n) uc[n] = u(i, j, k, n);
+    Prim p;
+    ctoprim_cell(uc, tab, p);
+    q(i, j, k, QRHO) = p.rho;
+    q(i, j, k, QU) = p.u;
+    q(i, j, k, QV) = p.v;
+    q(i, j, k, QW) = p.w;
+    q(i, j, k, QGAME) = p.game;
+    q(i, j, k, QPRES) = p.p;
+    q(i, j, k, QREINT) = p.rhoe;
+    q(i, j, k, QTEMP) = p.T;
 #pragma unroll
-    for (int n = 0; n < NSPEC; ++n) s2 = s2 + q(i, j, k, QFS + n) * zion(n) * (1.0 / aion(n));
-    const double abar = 1.0 / s1;
-    const double zbar = abar * s2;
-    const EosRE r = eos_re<true>(tab, rho, u(i, j, k, UTEMP), e, abar, zbar);
-    q(i, j, k, QTEMP) = r.T;
-    const double rhoe = e * rho;
-    q(i, j, k, QREINT) = rhoe;
-    q(i, j, k, QPRES) = r.p;
-    q(i, j, k, QGAME) = r.p / rhoe + 1.0;
-    qaux(i, j, k, QDPDR) = r.dpdr_e;
-    qaux(i, j, k, QDPDE) = r.dpde;
-    qaux(i, j, k, QGAMC) = r.gam1;
-    qaux(i, j, k, QC) = r.cs;
+    for (int n = 0; n < NSPEC; ++n) q(i, j, k, QFS + n) = p.x[n];
+    qaux(i, j, k, QGAMC) = p.gamc;
+    qaux(i, j, k, QC) = p.cs;
+    qaux(i, j, k, QDPDR) = p.dpdr_e;
+    qaux(i, j, k, QDPDE) = p.dpde;
 }
 
 // Source/Castro_hydro_nd.F90:11-69
@@ -488,45 +475,6 @@ __global__ __launch_bounds__(kBlock) void k_fill_hydro_source(BoxIter it, Arr sr
     }
 }
 
-// abar / zbar from species partial densities and 1/rho (Source/Castro_nd.F90:285-286)
-__device__ __forceinline__ void abar_zbar(const Arr &u, int i, int j, int k, double rhoInv, double &abar,
-                                          double &zbar)
-{
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) s1 = s1 + u(i, j, k, UFS + n) * (1.0 / aion(n));
-    abar = 1.0 / (s1 * rhoInv);
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) s2 = s2 + u(i, j, k, UFS + n) * zion(n) * (1.0 / aion(n));
-    zbar = abar * (s2 * rhoInv);
-}
-
-// Atomic min of a double through 64-bit compare-and-swap (vector memory).
-__device__ __forceinline__ void atomic_min_d(double *addr, double v)
-{
-    unsigned long long *a = reinterpret_cast<unsigned long long *>(addr);
-    unsigned long long old = *a;
-    while (v < __longlong_as_double(old)) {
-        const unsigned long long assumed = old;
-        old = atomicCAS(a, assumed, __double_as_longlong(v));
-        if (old == assumed) break;
-    }
-}
-
-__device__ __forceinline__ double block_min(double v)
-{
-    __shared__ double red[kBlock / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, red[w]);
-    }
-    return v;
-}
-
 // Source/timestep.F90:12-72 with the reduce_min of Source/reduction.F90:115-143
 __global__ __launch_bounds__(kBlock) void k_estdt(BoxIter it, Arr u, double dx0, double dx1, double dx2, double *dt,
                                                   EosTable tab)
@@ -535,103 +483,30 @@ __global__ __launch_bounds__(kBlock) void k_estdt(BoxIter it, Arr u, double dx0,
     int i, j, k;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (iter_cell(it, t, i, j, k)) {
-        const double rho = u(i, j, k, URHO);
-        const double rhoInv = 1.0 / rho;
-        double abar, zbar;
-        abar_zbar(u, i, j, k, rhoInv, abar, zbar);
-        const EosRE r = eos_re<true>(tab, rho, u(i, j, k, UTEMP), u(i, j, k, UEINT) * rhoInv, abar, zbar);
-        const double ux = u(i, j, k, UMX) * rhoInv, uy = u(i, j, k, UMY) * rhoInv, uz = u(i, j, k, UMZ) * rhoInv;
-        const double c = r.cs;
-        const double dt1 = dx0 / (c + fabs(ux));
-        const double dt2 = dx1 / (c + fabs(uy));
-        const double dt3 = dx2 / (c + fabs(uz));
-        v = fmin(fmin(dt1, dt2), dt3);
+        double uc[NVAR];
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) uc[n] = u(i, j, k, n);
+        v = estdt_cell(uc, dx0, dx1, dx2, tab);
     }
-    v = block_min(v);
+    v = block_min<kBlock>(v);
     if (threadIdx.x == 0) atomic_min_d(dt, v);
 }
 
-// Source/Castro_nd.F90:69-125
-__global__ __launch_bounds__(kBlock) void k_enforce_minimum_density(BoxIter it, Arr u, EosTable tab)
-{
-    MC_CELL(it)
-    const double rho0 = u(i, j, k, URHO);
-    if (!(rho0 < small_dens)) return;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) u(i, j, k, UFS + n) = u(i, j, k, UFS + n) * (small_dens / rho0);
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) s1 = s1 + u(i, j, k, UFS + n) * (1.0 / aion(n));
-    const double abar = 1.0 / (s1 / small_dens);
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) s2 = s2 + u(i, j, k, UFS + n) * zion(n) * (1.0 / aion(n));
-    const double zbar = abar * (s2 / small_dens);
-    const double e = eos_rt_e(tab, small_dens, small_temp, abar, zbar);
-    u(i, j, k, URHO) = small_dens;
-    u(i, j, k, UTEMP) = small_temp;
-    u(i, j, k, UMX) = 0.0;
-    u(i, j, k, UMY) = 0.0;
-    u(i, j, k, UMZ) = 0.0;
-    u(i, j, k, UEINT) = small_dens * e;
-    u(i, j, k, UEDEN) = u(i, j, k, UEINT);
-}
-
-// Source/Castro_nd.F90:129-160 (stores X, not rho X)
-__global__ __launch_bounds__(kBlock) void k_normalize_species(BoxIter it, Arr u)
-{
-    MC_CELL(it)
-    const double rho = u(i, j, k, URHO);
-    double x[NSPEC], sum = 0.0;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) x[n] = fmax(1.0e-30 * rho, fmin(rho, u(i, j, k, UFS + n)));
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) sum = sum + x[n];
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) u(i, j, k, UFS + n) = x[n] / sum;
-}
-
-// Source/Castro_nd.F90:164-249
-__global__ __launch_bounds__(kBlock) void k_reset_internal_e(BoxIter it, Arr u, EosTable tab)
-{
-    MC_CELL(it)
-    const double rho = u(i, j, k, URHO);
-    const double rhoInv = 1.0 / rho;
-    const double Up = u(i, j, k, UMX) * rhoInv, Vp = u(i, j, k, UMY) * rhoInv, Wp = u(i, j, k, UMZ) * rhoInv;
-    const double ke = HALF * (Up * Up + Vp * Vp + Wp * Wp);
-    const double E = u(i, j, k, UEDEN);
-    if (E < 0.0) {
-        if (u(i, j, k, UEINT) < 0.0) {
-            double abar, zbar;
-            abar_zbar(u, i, j, k, rhoInv, abar, zbar);
-            u(i, j, k, UEINT) = rho * eos_rt_e(tab, rho, small_temp, abar, zbar);
-        }
-        u(i, j, k, UEDEN) = u(i, j, k, UEINT) + rho * ke;
-    } else {
-        const double rho_eint = E - rho * ke;
-        if (rho_eint > 0.0 && rho_eint / E > 1.e-4) {
-            u(i, j, k, UEINT) = rho_eint;
-        } else if (u(i, j, k, UEINT) <= 0.0) {
-            double abar, zbar;
-            abar_zbar(u, i, j, k, rhoInv, abar, zbar);
-            const double eint_new = eos_rt_e(tab, rho, small_temp, abar, zbar);
-            u(i, j, k, UEINT) = rho * eint_new;
-        }
+#define MC_POINTWISE_KERNEL(kname, body)                                                  \
+    __global__ __launch_bounds__(kBlock) void kname(BoxIter it, Arr u, EosTable tab)      \
+    {                                                                                     \
+        MC_CELL(it)                                                                       \
+        double uc[NVAR];                                                                  \
+        _Pragma("unroll") for (int n = 0; n < NVAR; ++n) uc[n] = u(i, j, k, n);           \
+        body;                                                                             \
+        _Pragma("unroll") for (int n = 0; n < NVAR; ++n) u(i, j, k, n) = uc[n];           \
     }
-}
 
-// Source/Castro_nd.F90:253-297
-__global__ __launch_bounds__(kBlock) void k_compute_temp(BoxIter it, Arr u, EosTable tab)
-{
-    MC_CELL(it)
-    const double rho = u(i, j, k, URHO);
-    const double rhoInv = 1.0 / rho;
-    const double e = u(i, j, k, UEINT) * rhoInv;
-    double abar, zbar;
-    abar_zbar(u, i, j, k, rhoInv, abar, zbar);
-    const EosRE r = eos_re<false>(tab, rho, u(i, j, k, UTEMP), e, abar, zbar);
-    u(i, j, k, UTEMP) = r.T;
-    u(i, j, k, UEINT) = rho * e;
-}
+// Source/Castro_nd.F90:69-125, 129-160, 164-249, 253-297
+MC_POINTWISE_KERNEL(k_enforce_minimum_density, enforce_minimum_density_cell(uc, tab))
+MC_POINTWISE_KERNEL(k_normalize_species, (void)tab; normalize_species_cell(uc))
+MC_POINTWISE_KERNEL(k_reset_internal_e, reset_internal_e_cell(uc, tab))
+MC_POINTWISE_KERNEL(k_compute_temp, compute_temp_cell(uc, tab))
 
 // Source/Prob.F90:19-129 with the problem constants as arguments
 struct ProbDev {
@@ -908,7 +783,7 @@ void enforce_minimum_density(const int *lo, const int *hi, double *state, const 
 void normalize_species(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi)
 {
     BoxIter it = make_iter(make_box(lo, hi));
-    hipLaunchKernelGGL(k_normalize_species, dim3(grid_for(it.n)), dim3(kBlock), 0, t_stream, it, ARR(state));
+    hipLaunchKernelGGL(k_normalize_species, dim3(grid_for(it.n)), dim3(kBlock), 0, t_stream, it, ARR(state), g_eos);
 }
 
 void reset_internal_e(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi)

@@ -108,6 +108,7 @@ struct mc_level {
     int steps = 0;
     double time = 0.0, dt = 0.0;
     FusedLevel *fused = nullptr;
+    bool fused_ahead = false;  // the fused engine's state is newer than S_new
 };
 
 static double *dalloc(size_t n, bool &ok)
@@ -302,27 +303,84 @@ void mc_level_destroy(mc_level_t *L)
     delete L;
 }
 
+static int sync_from_fused(mc_level_t *L)
+{
+    if (!L->fused_ahead) return 0;
+    if (fused_export(L->fused, L->S_new, L->stream)) return -1;
+    L->fused_ahead = false;
+    return check_hip(hipStreamSynchronize(L->stream), "fused export sync");
+}
+
+// Hand S_new and the step clock to the fused engine.
+static int sync_to_fused(mc_level_t *L)
+{
+    if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+    if (fused_import(L->fused, L->S_new, L->stream)) return -1;
+    if (fused_set_clock(L->fused, 0, L->dt, L->time, L->steps, L->stream)) return -1;
+    if (fused_estimate(L->fused, L->stream)) return -1;
+    L->fused_ahead = true;
+    return 0;
+}
+
 int mc_level_set_mode(mc_level_t *L, int mode)
 {
     if (mode != MC_MODE_PLAIN && mode != MC_MODE_FUSED) {
         set_error("mc_level_set_mode: unknown mode");
         return -1;
     }
+    if (mode == L->mode) return 0;
+    if (mode == MC_MODE_PLAIN && sync_from_fused(L)) return -1;
     L->mode = mode;
+    if (mode == MC_MODE_FUSED && eos_ready()) return sync_to_fused(L);
     return 0;
+}
+
+int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
+{
+    if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+    if (stage < 0 || stage >= FST_N || bx <= 0 || by <= 0 || bz <= 0 || (bx * by * bz) % 64 ||
+        bx * by * bz > 256) {
+        set_error("mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256");
+        return -1;
+    }
+    fused_set_tiling(L->fused, stage, bx, by, bz);
+    return 0;
+}
+
+int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz)
+{
+    if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+    if (stage < 0 || stage >= FST_N) return -1;
+    fused_get_tiling(L->fused, stage, bxyz);
+    return 0;
+}
+
+int mc_level_run_stage(mc_level_t *L, int stage)
+{
+    if (L->mode != MC_MODE_FUSED || !L->fused) {
+        set_error("mc_level_run_stage: fused mode only");
+        return -1;
+    }
+    return fused_run_stage(L->fused, stage, L->stream);
 }
 
 int mc_level_nbox(const mc_level_t *L) { return (int)L->boxes.size(); }
 
-double *mc_level_state_ptr(mc_level_t *L, int b) { return L->S_new + L->off_valid[b]; }
+double *mc_level_state_ptr(mc_level_t *L, int b)
+{
+    if (sync_from_fused(L)) return nullptr;
+    return L->S_new + L->off_valid[b];
+}
 
 int mc_level_set_state(mc_level_t *L, int b, const double *host)
 {
     if (b < 0 || b >= (int)L->boxes.size()) return -1;
     const long n = box_npts(L->boxes[b]) * NVAR;
+    if (sync_from_fused(L)) return -1;
     if (check_hip(hipMemcpyAsync(L->S_new + L->off_valid[b], host, n * sizeof(double), hipMemcpyHostToDevice,
                                  L->stream), "set_state"))
         return -1;
+    if (L->mode == MC_MODE_FUSED && sync_to_fused(L)) return -1;
     return check_hip(hipStreamSynchronize(L->stream), "set_state sync");
 }
 
@@ -330,6 +388,7 @@ int mc_level_get_state(mc_level_t *L, int b, double *host)
 {
     if (b < 0 || b >= (int)L->boxes.size()) return -1;
     const long n = box_npts(L->boxes[b]) * NVAR;
+    if (sync_from_fused(L)) return -1;
     if (check_hip(hipMemcpyAsync(host, L->S_new + L->off_valid[b], n * sizeof(double), hipMemcpyDeviceToHost,
                                  L->stream), "get_state"))
         return -1;
@@ -357,6 +416,8 @@ int mc_level_init_data(mc_level_t *L, double p_ambient, double dens_ambient, dou
     L->steps = 0;
     L->time = 0.0;
     L->dt = 0.0;
+    L->fused_ahead = false;
+    if (L->mode == MC_MODE_FUSED && sync_to_fused(L)) return -1;
     return check_hip(hipStreamSynchronize(L->stream), "init_data");
 }
 
@@ -473,8 +534,9 @@ double mc_level_est_time_step(mc_level_t *L)
     }
     double dt = DBL_MAX;
     if (L->mode == MC_MODE_FUSED) {
-        if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1.0;
-        if (fused_est_dt(L->fused, L->S_new, L->dt_dev, L->stream)) return -1.0;
+        if (!L->fused_ahead && sync_to_fused(L)) return -1.0;
+        if (fused_estimate(L->fused, L->stream) || fused_read_est(L->fused, 0, &dt, L->stream)) return -1.0;
+        return dt * cfl;
     } else {
         fill_const(L->stream, L->dt_dev, 1, DBL_MAX);
         for (size_t b = 0; b < L->boxes.size(); ++b)
@@ -500,11 +562,12 @@ int mc_level_advance(mc_level_t *L, double dt)
         set_error("mc_level_advance: EOS table not initialised");
         return -1;
     }
-    std::swap(L->S_old, L->S_new); /* swapTimeLevels, Castro_advance.cpp:32-33 */
     if (L->mode == MC_MODE_FUSED) {
-        if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
-        return fused_advance(L->fused, L->S_old, L->S_new, dt, L->stream);
+        // the fused update includes post_timestep's clean_state
+        if (!L->fused_ahead && sync_to_fused(L)) return -1;
+        return fused_advance_dt(L->fused, dt, L->stream);
     }
+    std::swap(L->S_old, L->S_new); /* swapTimeLevels, Castro_advance.cpp:32-33 */
     if (alloc_plain(L)) return -1;
     void *prev = mc_get_stream();
     mc_set_stream(L->stream);
@@ -524,7 +587,7 @@ int mc_level_advance(mc_level_t *L, double dt)
 
 int mc_level_post_timestep(mc_level_t *L)
 {
-    if (L->mode == MC_MODE_FUSED) return fused_clean(L->fused, L->S_new, L->stream);
+    if (L->mode == MC_MODE_FUSED) return 0; /* folded into the fused update */
     void *prev = mc_get_stream();
     mc_set_stream(L->stream);
     clean_state(L, L->S_new, L->off_valid, 0); /* Castro.cpp:369 */
@@ -532,8 +595,21 @@ int mc_level_post_timestep(mc_level_t *L)
     return check_hip(hipGetLastError(), "post_timestep");
 }
 
+int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph)
+{
+    if (L->mode == MC_MODE_FUSED) {
+        if (!L->fused_ahead && sync_to_fused(L)) return -1;
+        if (fused_steps(L->fused, nsteps, stop_time, L->stream, use_graph != 0)) return -1;
+        return fused_read_clock(L->fused, 0, &L->dt, &L->time, &L->steps, L->stream);
+    }
+    for (int n = 0; n < nsteps; ++n)
+        if (mc_level_coarse_step(L, stop_time) < 0) return -1;
+    return 0;
+}
+
 double mc_level_coarse_step(mc_level_t *L, double stop_time)
 {
+    if (L->mode == MC_MODE_FUSED) return mc_level_run(L, 1, stop_time, 0) ? -1.0 : L->dt;
     double dt_0;
     if (L->steps > 0) { /* Castro.cpp:218-295 */
         const double change_max = 1.1e0;

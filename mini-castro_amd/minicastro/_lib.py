@@ -78,6 +78,10 @@ SIGS = [
     ("mc_level_time", _D, [_vp]),
     ("mc_level_dt", _D, [_vp]),
     ("mc_level_sync", _I, [_vp]),
+    ("mc_level_run", _I, [_vp, _I, _D, _I]),
+    ("mc_level_set_tiling", _I, [_vp, _I, _I, _I, _I]),
+    ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
+    ("mc_level_run_stage", _I, [_vp, _I]),
 ]
 
 _lib = None

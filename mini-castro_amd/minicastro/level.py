@@ -85,6 +85,22 @@ class Level:
             raise RuntimeError("mc_level_coarse_step: %s" % self.L.mc_last_error().decode())
         return v
 
+    def run(self, nsteps, stop_time=-1.0, use_graph=True):
+        """nsteps coarse steps; in fused mode dt stays on the device and the
+        step is replayed from a captured hipGraph."""
+        check(self.L.mc_level_run(self.h, int(nsteps), stop_time, 1 if use_graph else 0), "mc_level_run")
+
+    def set_tiling(self, stage, bx, by, bz):
+        check(self.L.mc_level_set_tiling(self.h, int(stage), bx, by, bz), "mc_level_set_tiling")
+
+    def get_tiling(self, stage):
+        a = (C.c_int * 3)()
+        check(self.L.mc_level_get_tiling(self.h, int(stage), a), "mc_level_get_tiling")
+        return tuple(a)
+
+    def run_stage(self, stage):
+        check(self.L.mc_level_run_stage(self.h, int(stage)), "mc_level_run_stage")
+
     def sync(self):
         check(self.L.mc_level_sync(self.h), "mc_level_sync")
 

@@ -1,8 +1,14 @@
 """Benchmark: zone-updates per second of the CTU Sedov step on one MI355X.
 
-A step is one Amr::coarseTimeStep (computeNewDt with the estdt reduction,
-Castro::advance, post_timestep) over every box of the level, inputs resident
-in HBM.  Prints one JSON line.
+A step is one Amr::coarseTimeStep of mini-Castro (time-step control with the
+estdt reduction, Castro::advance, post_timestep) over every box of the
+level, with the state resident in HBM.  Default workload: 256^3 Sedov tiled
+by 64 seeded uneven boxes (sides in {16,...,96}), batched fused path.  The
+line also carries the per-box (reference kernel order) rate on the same
+workload, the roofline of the dominant stage measured with HIP events on the
+stream it runs on, and the CPU oracle timed on a bounded sample.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -15,43 +21,134 @@ for p in (os.path.join(ROOT, "mini-castro_amd"), os.path.join(ROOT, "oracle"), o
     if p not in sys.path:
         sys.path.insert(0, p)
 
+METRIC = "zones/s at 256^3 Sedov FP64, batched vs per-box (1e-12 rel); adjoint/fwd time"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+# Algorithmic HBM bytes per unit of each fused stage: every input element
+# read once and every output element written once (doubles).
+#   unit = a cell of the domain (prim, update), a cell of the one-cell ring
+#   (flat, trace per direction), a face (flux1, final per direction,
+#   trans1 per direction pair)
+STAGE_BYTES = {
+    "prim": (20 + 20 + 21) * 8,          # U in; U1, Q out
+    "flat": (4 + 2) * 8,                 # p,u,v,w in; flatn, div out
+    "trace": (21 + 1 + 2 * 19) * 8,      # Q, flatn in; minus/plus states out
+    "flux1": (2 * 19 + 2 + 20) * 8,      # states, c/gamc in; flux + Godunov out
+    "trans1": (2 * 19 + 20 + 1 + 20) * 8,  # states, d-flux, gamc in; flux out
+    "final": (2 * 19 + 2 * 20 + 1 + 20 + 2 + 22) * 8,  # states, 2 fluxes, div, U1, c/gamc in; flux out
+    "update": (3 * 22 + 20 + 20) * 8,    # 3 final fluxes, U in; U out
+}
+STAGE_UNITS_PER_LAUNCH = {  # (stage) -> function of n_cell giving units per launch
+    "prim": lambda n: n[0] * n[1] * n[2],
+    "update": lambda n: n[0] * n[1] * n[2],
+    "flat": lambda n: (n[0] + 2) * (n[1] + 2) * (n[2] + 2),
+    "trace": lambda n: 3 * (n[0] + 2) * (n[1] + 2) * (n[2] + 2),
+    "flux1": lambda n: 3 * (n[0] + 1) * (n[1] + 2) * (n[2] + 2),
+    "trans1": lambda n: 6 * (n[0] + 1) * n[1] * (n[2] + 2),
+    "final": lambda n: 3 * (n[0] + 1) * n[1] * n[2],
+}
+
+
+def cpu_baseline(cfg_full, n=64, steps=2):
+    """The oracle (CPU restatement, OpenMP over (1024,16,16) tiles) on an n^3
+    single-box sample of the same problem; zone-updates/s."""
+    import numpy as np
+    import orc
+    from helpers import oracle_level
+    from minicastro import configs
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cfg = dict(cfg_full)
+    cfg.update(n_cell=(n, n, n), boxes=configs.grid_boxes(n, n))
+    L = oracle_level(cfg, nthreads=threads)
+    L.coarse_step()  # first step (the initial-temperature Newton solves are longer); untimed
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.coarse_step()
+    t1 = time.perf_counter()
+    del np, orc
+    return {"value": n ** 3 * steps / (t1 - t0), "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d^3 single-box Sedov (same problem constants), %d steps after one untimed step, "
+                      "oracle OpenMP over (1024,16,16) tiles" % (n, steps)}
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain", "fused"])
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3, help="per-box comparison steps (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU; this build does not run across GPUs (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import torch
     import minicastro as mc
     from minicastro import configs
 
     cfg = configs.config(a.config)
+    n = cfg["n_cell"]
+    zones = int(n[0]) * int(n[1]) * int(n[2])
     mc.load_table()
-    mode = mc.MODE_FUSED if a.mode == "fused" else mc.MODE_PLAIN
-    L = mc.Level(cfg["n_cell"], cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mode)
+
+    # batched fused path
+    L = mc.Level(n, cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_FUSED)
     L.init_data(**cfg)
-    L.run(a.warmup)
+    L.run(a.warmup, use_graph=True)
     L.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    L.run(a.steps)
+    stages = L.run_timed(a.steps)
     L.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    zones = int(cfg["n_cell"][0]) * int(cfg["n_cell"][1]) * int(cfg["n_cell"][2])
-    ms = (t1 - t0) * 1e3 / a.steps
-    out = {"metric": "zones/s at 256^3 Sedov FP64, batched vs per-box (1e-12 rel); adjoint/fwd time",
-           "value": zones * a.steps / (t1 - t0), "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    value = zones * a.steps / (t1 - t0)
+    # graph replay rate (dt on the device, no host work between steps)
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    L.run(a.steps, use_graph=True)
+    L.sync()
+    g1 = time.perf_counter()
+    graph_ms = (g1 - g0) * 1e3 / a.steps
+    fused_steps_done = L.steps
+    L.close()
+
+    dom = max((k for k in stages if k in STAGE_BYTES), key=lambda k: stages[k][0])
+    tot_ms, nl = stages[dom]
+    avg_s = tot_ms / nl / 1e3
+    traffic_alg = STAGE_BYTES[dom] * STAGE_UNITS_PER_LAUNCH[dom](n)
+    achieved = traffic_alg / avg_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "mc::K_" + dom,
+                "avg_launch_ms": tot_ms / nl, "algorithmic_bytes_per_launch": traffic_alg}
+
+    # per-box path (the reference's kernel order) on the same workload
+    plain = None
+    if a.plain_steps > 0:
+        P = mc.Level(n, cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_PLAIN)
+        P.init_data(**cfg)
+        P.run(1)
+        P.sync()
+        p0 = time.perf_counter()
+        P.run(a.plain_steps)
+        P.sync()
+        p1 = time.perf_counter()
+        plain = (p1 - p0) * 1e3 / a.plain_steps
+        P.close()
+
+    cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
+    out = {"metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": cfg["name"], "n_cell": list(cfg["n_cell"]), "boxes": len(cfg["boxes"]),
-                      "mode": a.mode}}
+           "config": {"workload": cfg["name"], "n_cell": list(n), "boxes": int(len(cfg["boxes"])),
+                      "path": "batched fused, one launch per stage over all boxes",
+                      "graph_replay_ms_per_step": graph_ms, "steps_advanced": fused_steps_done,
+                      "per_box_ms_per_step": plain,
+                      "batched_vs_per_box": (plain / ms_step) if plain else None,
+                      "stage_ms_per_step": {k: v[0] / a.steps for k, v in stages.items() if v[1]}},
+           "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

@@ -220,6 +220,10 @@ double mc_level_coarse_step(mc_level_t *L, double stop_time);
 /* nsteps coarse steps; in fused mode dt stays on the device between steps and
    the step may be replayed from a captured hipGraph (use_graph != 0). */
 int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph);
+/* Fused mode: nsteps eager steps with HIP events around every stage launch on
+   the level's stream; stage_ms[0..8] / launches[0..8] receive per-stage totals
+   (index 8: the time-step control kernels). */
+int mc_level_run_timed(mc_level_t *L, int nsteps, double stop_time, double *stage_ms, int *launches);
 int mc_level_steps(const mc_level_t *L);
 double mc_level_time(const mc_level_t *L);
 double mc_level_dt(const mc_level_t *L);

@@ -921,6 +921,50 @@ static void enqueue_step(FusedLevel *F, double stop_time, hipStream_t s)
     hipLaunchKernelGGL(K_tick, dim3(1), dim3(256), 0, s, F->ctx, F->ninst);
 }
 
+// Eager steps with a HIP event pair around every stage launch on stream s;
+// stage_ms[st] receives the summed duration of stage st, launches[st] the
+// number of launches (the dt kernels are counted under FST_N).
+int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, double *stage_ms, int *launches)
+{
+    if (!F->have_est && fused_estimate(F, s)) return -1;
+    static const int order[7] = {ST_PRIM, ST_FLAT, ST_TRACE, ST_FLUX1, ST_TRANS1, ST_FINAL, ST_UPDATE};
+    static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    const int per = 8;  // 7 stages + the dt kernels
+    std::vector<hipEvent_t> ev(2 * per * (size_t)nsteps);
+    for (auto &e : ev)
+        if (check_hip(hipEventCreate(&e), "hipEventCreate")) return -1;
+    F->ctx.tab = eos_table();
+    for (int n = 0; n < nsteps; ++n) {
+        hipEvent_t *E = ev.data() + 2 * per * (size_t)n;
+        (void)hipEventRecord(E[14], s);
+        hipLaunchKernelGGL(K_dt, dim3(1), dim3(256), 0, s, F->ctx, F->ninst, stop_time);
+        (void)hipEventRecord(E[15], s);
+        for (int q = 0; q < 7; ++q) {
+            (void)hipEventRecord(E[2 * q], s);
+            launch_stage(F, order[q], s, ny[order[q]]);
+            (void)hipEventRecord(E[2 * q + 1], s);
+        }
+        hipLaunchKernelGGL(K_tick, dim3(1), dim3(256), 0, s, F->ctx, F->ninst);
+    }
+    int rc = check_hip(hipStreamSynchronize(s), "timed steps");
+    for (int st = 0; st <= NSTAGE; ++st) {
+        stage_ms[st] = 0.0;
+        launches[st] = 0;
+    }
+    for (int n = 0; n < nsteps && rc == 0; ++n) {
+        hipEvent_t *E = ev.data() + 2 * per * (size_t)n;
+        for (int q = 0; q < 8; ++q) {
+            float ms = 0.f;
+            if (check_hip(hipEventElapsedTime(&ms, E[2 * q], E[2 * q + 1]), "hipEventElapsedTime")) rc = -1;
+            const int st = q < 7 ? order[q] : NSTAGE;
+            stage_ms[st] += ms;
+            launches[st] += 1;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
 int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool use_graph)
 {
     if (!F->have_est && fused_estimate(F, s)) return -1;

@@ -607,6 +607,17 @@ int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph)
     return 0;
 }
 
+int mc_level_run_timed(mc_level_t *L, int nsteps, double stop_time, double *stage_ms, int *launches)
+{
+    if (L->mode != MC_MODE_FUSED) {
+        set_error("mc_level_run_timed: fused mode only");
+        return -1;
+    }
+    if (!L->fused_ahead && sync_to_fused(L)) return -1;
+    if (fused_steps_timed(L->fused, nsteps, stop_time, L->stream, stage_ms, launches)) return -1;
+    return fused_read_clock(L->fused, 0, &L->dt, &L->time, &L->steps, L->stream);
+}
+
 double mc_level_coarse_step(mc_level_t *L, double stop_time)
 {
     if (L->mode == MC_MODE_FUSED) return mc_level_run(L, 1, stop_time, 0) ? -1.0 : L->dt;

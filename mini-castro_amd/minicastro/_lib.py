@@ -82,6 +82,7 @@ SIGS = [
     ("mc_level_set_tiling", _I, [_vp, _I, _I, _I, _I]),
     ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
     ("mc_level_run_stage", _I, [_vp, _I]),
+    ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
 ]
 
 _lib = None

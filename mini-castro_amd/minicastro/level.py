@@ -90,6 +90,16 @@ class Level:
         step is replayed from a captured hipGraph."""
         check(self.L.mc_level_run(self.h, int(nsteps), stop_time, 1 if use_graph else 0), "mc_level_run")
 
+    STAGES = ("est", "prim", "flat", "trace", "flux1", "trans1", "final", "update", "dt")
+
+    def run_timed(self, nsteps, stop_time=-1.0):
+        """Fused mode: nsteps eager steps, HIP events around each stage launch.
+        Returns {stage: (total_ms, launches)}."""
+        ms = (C.c_double * 9)()
+        nl = (C.c_int * 9)()
+        check(self.L.mc_level_run_timed(self.h, int(nsteps), stop_time, ms, nl), "mc_level_run_timed")
+        return {self.STAGES[i]: (ms[i], nl[i]) for i in range(9)}
+
     def set_tiling(self, stage, bx, by, bz):
         check(self.L.mc_level_set_tiling(self.h, int(stage), bx, by, bz), "mc_level_set_tiling")
 

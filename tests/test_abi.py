@@ -1,0 +1,59 @@
+"""The C ABI library loads without a GPU and exports every entry point that
+include/mc_castro.h declares (the names a mini-Castro host binds,
+Source/Castro_F.H:11-161, plus the mc_ extensions).  No compute calls."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import minicastro as mc
+
+HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mc_castro.h")
+REFERENCE_NAMES = ["network_init", "network_finalize", "eos_init", "eos_finalize", "ctoprim",
+                   "enforce_minimum_density", "normalize_species", "reset_internal_e", "compute_temp", "estdt",
+                   "store_flux", "fill_hydro_source", "normalize_species_fluxes", "apply_av", "trans1", "trans2",
+                   "compute_flux", "divu", "trace_ppm", "initdata", "denerror", "calculate_blast_radius"]
+
+
+def declared():
+    src = open(HDR).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", src, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "sizeof")))
+
+
+def test_header_declares_reference_entry_points():
+    d = declared()
+    for n in REFERENCE_NAMES:
+        assert n in d, n
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(mc.lib_path())
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    bound = {n for n, _, _ in mc._lib.SIGS}
+    assert set(declared()) <= bound, set(declared()) - bound
+
+
+def test_errors_without_table_are_reported_not_fatal():
+    L = mc.lib()
+    lvl = L.mc_level_create((C.c_int * 3)(8, 8, 8), (C.c_double * 3)(0, 0, 0), (C.c_double * 3)(1, 1, 1), 1, 1,
+                            (C.c_int * 6)(0, 0, 0, 7, 7, 7))
+    if not lvl:
+        pytest.skip("no GPU runtime: level allocation fails")
+    L.mc_level_destroy(lvl)
+
+
+def test_bad_box_lists_rejected():
+    L = mc.lib()
+    mk = lambda boxes: L.mc_level_create((C.c_int * 3)(8, 8, 8), (C.c_double * 3)(0, 0, 0),
+                                         (C.c_double * 3)(1, 1, 1), 1, len(boxes) // 6, (C.c_int * len(boxes))(*boxes))
+    assert not mk([0, 0, 0, 8, 7, 7])            # outside the domain
+    assert not mk([0, 0, 0, 3, 7, 7])            # does not tile the domain
+    assert not mk([0, 0, 0, 7, 7, 7, 0, 0, 0, 7, 7, 7])  # overlap / double count
+    assert b"tile" in L.mc_last_error() or b"domain" in L.mc_last_error() or b"overlap" in L.mc_last_error()

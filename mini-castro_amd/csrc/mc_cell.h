@@ -179,8 +179,9 @@ __device__ __forceinline__ void atomic_min_d(double *addr, double v)
     }
 }
 
-// Minimum over a block whose size is a multiple of 64 (wavefront shuffles,
-// then one LDS word per wave); the result is valid in thread 0.
+// Minimum over a 1-D block of blockDim.x <= NT threads, blockDim.x a
+// multiple of 64 (wavefront shuffles, then one LDS word per wave); the
+// result is valid in thread 0.
 template <int NT>
 __device__ __forceinline__ double block_min(double v)
 {
@@ -191,8 +192,8 @@ __device__ __forceinline__ double block_min(double v)
     if (lane == 0) red[wv] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int w = 1; w < NT / 64; ++w) v = fmin(v, red[w]);
+        const int nw = (int)(blockDim.x >> 6);
+        for (int w = 1; w < nw; ++w) v = fmin(v, red[w]);
     }
     return v;
 }

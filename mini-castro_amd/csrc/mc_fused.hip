@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     }
 }
 
-__device__ __forceinline__ void load_fs(double *S, const FInst &I, long e, int d, Face &f)
+// Hydro part (rho, velocities permuted to direction d, p, rhoe) of a state
+__device__ __forceinline__ void load_fs_h(double *S, const FInst &I, long e, int d, FaceH &f)
 {
     const int un = FS_U + d, ut = d == 0 ? FS_V : FS_U, utt = d == 2 ? FS_V : FS_W;
     f.rho = EA(S, I, NFS, FS_RHO, e);
@@ -330,24 +331,33 @@ __device__ __forceinline__ void load_fs(double *S, const FInst &I, long e, int d
     f.utt = EA(S, I, NFS, utt, e);
     f.p = EA(S, I, NFS, FS_P, e);
     f.rhoe = EA(S, I, NFS, FS_RE, e);
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) f.x[n] = EA(S, I, NFS, FS_X + n, e);
 }
 
-__device__ __forceinline__ void qstate_to_face(const QState &o, int d, Face &f)
+__device__ __forceinline__ void load_qin_h(double *S, const FInst &I, long e, QInH &q)
 {
+    q.rho = EA(S, I, NFS, FS_RHO, e);
+    q.u = EA(S, I, NFS, FS_U, e);
+    q.v = EA(S, I, NFS, FS_V, e);
+    q.w = EA(S, I, NFS, FS_W, e);
+    q.p = EA(S, I, NFS, FS_P, e);
+    q.rhoe = EA(S, I, NFS, FS_RE, e);
+    q.game = 0.0;
+}
+
+__device__ __forceinline__ FaceH to_face(const QStateH &o, int d)
+{
+    FaceH f;
     f.rho = o.rho;
     f.un = d == 0 ? o.u : (d == 1 ? o.v : o.w);
     f.ut = d == 0 ? o.v : o.u;
     f.utt = d == 2 ? o.v : o.w;
     f.p = o.p;
     f.rhoe = o.rhoe;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) f.x[n] = o.x[n];
+    return f;
 }
 
-// Fluxes in the transverse-stage layout; momenta back in absolute order.
-__device__ __forceinline__ void store_tflux(double *F, const FInst &I, long e, int d, const RiemannOut &o)
+// Non-species part of a transverse-stage flux; momenta in absolute order.
+__device__ __forceinline__ void store_tflux_h(double *F, const FInst &I, long e, int d, const RiemannH &o)
 {
     const int mn = FF_MX + d, mt = d == 0 ? FF_MY : FF_MX, mtt = d == 2 ? FF_MY : FF_MZ;
     EA(F, I, NFF, FF_RHO, e) = o.f_rho;
@@ -355,24 +365,22 @@ __device__ __forceinline__ void store_tflux(double *F, const FInst &I, long e, i
     EA(F, I, NFF, mt, e) = o.f_mt;
     EA(F, I, NFF, mtt, e) = o.f_mtt;
     EA(F, I, NFF, FF_E, e) = o.f_E;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) EA(F, I, NFF, FF_X + n, e) = o.f_rho * o.gx[n];
     EA(F, I, NFF, FF_GP, e) = o.g_p;
     EA(F, I, NFF, FF_GU, e) = o.g_un;
 }
 
-// Riemann problem at face e (cell index of its high side (i,j,k)) in direction d
-__device__ __forceinline__ void face_riemann(const Ctx &x, const FInst &I, int d, int i, int j, int k, const Face &L,
-                                             const Face &R, RiemannOut &o)
+// Riemann problem on face (i,j,k) of direction d (cells i-e_d and i)
+__device__ __forceinline__ void face_riemann_h(const Ctx &x, const FInst &I, int d, int i, int j, int k,
+                                               const FaceH &L, const FaceH &R, RiemannH &o)
 {
     const long cr = cidx(I, i, j, k);
     const long cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
-    riemann_cgf(L, R, CA(x.Q, I, NFQ, FQ_C, cl), CA(x.Q, I, NFQ, FQ_C, cr), CA(x.Q, I, NFQ, FQ_G, cl),
-                CA(x.Q, I, NFQ, FQ_G, cr), o);
+    riemann_hydro(L, R, CA(x.Q, I, NFQ, FQ_C, cl), CA(x.Q, I, NFQ, FQ_C, cr), CA(x.Q, I, NFQ, FQ_G, cl),
+                  CA(x.Q, I, NFQ, FQ_G, cr), o);
 }
 
 // Source/riemann.F90:10-331 on the PPM states: F^d on d-faces
-// (d-face index 0..n_d, transverse cells -1..n)
+// (d-face index 0..n_d, transverse cells -1..n).  Species are streamed.
 __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
 {
     int m, i, j, k;
@@ -382,34 +390,25 @@ __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     if (pos < 0) return;
     const long e = eidx(I, i, j, k);
-    Face L, R;
-    load_fs(x.SM[d], I, e, d, L);
-    load_fs(x.SP[d], I, e, d, R);
-    RiemannOut o;
-    face_riemann(x, I, d, i, j, k, L, R, o);
-    store_tflux(x.FD[d], I, e, d, o);
+    FaceH L, R;
+    load_fs_h(x.SM[d], I, e, d, L);
+    load_fs_h(x.SP[d], I, e, d, R);
+    RiemannH o;
+    face_riemann_h(x, I, d, i, j, k, L, R, o);
+    store_tflux_h(x.FD[d], I, e, d, o);
+    double *F = x.FD[d];
+#pragma unroll 1
+    for (int n = 0; n < NSPEC; ++n) {
+        const double gx = o.fpl * EA(x.SM[d], I, NFS, FS_X + n, e) + o.fmr * EA(x.SP[d], I, NFS, FS_X + n, e);
+        EA(F, I, NFF, FF_X + n, e) = o.f_rho * gx;
+    }
 }
 
-__device__ __forceinline__ void load_qin_fs(double *S, const FInst &I, long e, QIn &q)
-{
-    q.rho = EA(S, I, NFS, FS_RHO, e);
-    q.u = EA(S, I, NFS, FS_U, e);
-    q.v = EA(S, I, NFS, FS_V, e);
-    q.w = EA(S, I, NFS, FS_W, e);
-    q.p = EA(S, I, NFS, FS_P, e);
-    q.rhoe = EA(S, I, NFS, FS_RE, e);
-    q.game = 0.0;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) q.x[n] = EA(S, I, NFS, FS_X + n, e);
-}
-
-// flux components a transverse update reads, in conserved-component slots
-__device__ __forceinline__ void load_tflux(double *F, const FInst &I, long e, double *v, double &gp, double &gu)
+// URHO..UEDEN of a transverse-stage flux and its Godunov p, u
+__device__ __forceinline__ void load_tflux_h(double *F, const FInst &I, long e, double *v, double &gp, double &gu)
 {
 #pragma unroll
     for (int n = 0; n <= UEDEN; ++n) v[n] = EA(F, I, NFF, FF_RHO + n, e);
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) v[UFS + n] = EA(F, I, NFF, FF_X + n, e);
     gp = EA(F, I, NFF, FF_GP, e);
     gu = EA(F, I, NFF, FF_GU, e);
 }
@@ -418,7 +417,7 @@ __constant__ int c_pair_t[6] = {1, 2, 0, 2, 0, 1};   // state (face) direction
 __constant__ int c_pair_d[6] = {0, 0, 1, 1, 2, 2};   // flux direction
 
 // trans1 (Source/trans.F90:14-243) of the t-face states by the d-flux, then
-// the flux F^{t|d} of the corrected states on the t-faces of tbx[t][d]
+// the flux F^{t|d} of the corrected states on the t-faces of tbx[t][d].
 __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
 {
     int m, i, j, k;
@@ -431,23 +430,40 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
     if (c[t] < 0 || c[d] < 0 || c[d] > I.n[d] - 1) return;
     const long e = eidx(I, i, j, k);
     const int et[3] = {t == 0, t == 1, t == 2}, ed[3] = {d == 0, d == 1, d == 2};
-    const double cdtdx = x.dt_cur[m] / I.dx[d] / 3.0;
-    Face side[2];
+    const double cdtdx = x.dt_cur[m] / I.dx[d];
+    const double cdt = cdtdx / 3.0;
+    FaceH side[2];
+    long el[2], er[2];
+    double rho[2], rrnew[2];
+#pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int sh = s - 1;  // -1: minus state, 0: plus state
+        const int sh = s - 1;  // -1: minus state (zone t-1), 0: plus state (zone t)
         const int il = i + sh * et[0], jl = j + sh * et[1], kl = k + sh * et[2];
-        QIn lq;
-        load_qin_fs(s == 0 ? x.SM[t] : x.SP[t], I, e, lq);
-        double fr[NVAR], fl[NVAR], pgp, ugp, pgm, ugm;
-        load_tflux(x.FD[d], I, eidx(I, il + ed[0], jl + ed[1], kl + ed[2]), fr, pgp, ugp);
-        load_tflux(x.FD[d], I, eidx(I, il, jl, kl), fl, pgm, ugm);
+        el[s] = eidx(I, il, jl, kl);
+        er[s] = eidx(I, il + ed[0], jl + ed[1], kl + ed[2]);
+        QInH lq;
+        load_qin_h(s == 0 ? x.SM[t] : x.SP[t], I, e, lq);
+        double fr[UEDEN + 1], fl[UEDEN + 1], pgp, ugp, pgm, ugm;
+        load_tflux_h(x.FD[d], I, er[s], fr, pgp, ugp);
+        load_tflux_h(x.FD[d], I, el[s], fl, pgm, ugm);
         const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, il, jl, kl));
-        const QState o = trans1_state(lq, fr, fl, pgp, pgm, ugp, ugm, gamc, cdtdx);
-        qstate_to_face(o, t, side[s]);
+        const QStateH o = trans1_hydro(lq, fr, fl, pgp, pgm, ugp, ugm, gamc, cdt);
+        rho[s] = lq.rho;
+        rrnew[s] = lq.rho - cdt * (fr[URHO] - fl[URHO]);
+        side[s] = to_face(o, t);
     }
-    RiemannOut o;
-    face_riemann(x, I, t, i, j, k, side[0], side[1], o);
-    store_tflux(x.FT[combo], I, e, t, o);
+    RiemannH o;
+    face_riemann_h(x, I, t, i, j, k, side[0], side[1], o);
+    double *F = x.FT[combo], *FDd = x.FD[d];
+    store_tflux_h(F, I, e, t, o);
+#pragma unroll 1
+    for (int n = 0; n < NSPEC; ++n) {
+        const double x0 = trans1_species(rho[0], EA(x.SM[t], I, NFS, FS_X + n, e), EA(FDd, I, NFF, FF_X + n, er[0]),
+                                         EA(FDd, I, NFF, FF_X + n, el[0]), cdt, rrnew[0]);
+        const double x1 = trans1_species(rho[1], EA(x.SP[t], I, NFS, FS_X + n, e), EA(FDd, I, NFF, FF_X + n, er[1]),
+                                         EA(FDd, I, NFF, FF_X + n, el[1]), cdt, rrnew[1]);
+        EA(F, I, NFF, FF_X + n, e) = o.f_rho * (o.fpl * x0 + o.fmr * x1);
+    }
 }
 
 // index of the F^{a|b} array
@@ -459,7 +475,7 @@ __device__ __forceinline__ int pair_index(int a, int b)
 
 // trans2 (Source/trans.F90:250-505) + final flux + apply_av
 // (Source/Castro_hydro_nd.F90:174-245) + normalize_species_fluxes (:248-297)
-// on the d-faces of the valid region
+// on the d-faces of the valid region.
 __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 {
     int m, i, j, k;
@@ -477,35 +493,33 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     const double h2 = 0.5 * dt / I.dx[t1], h3 = 0.5 * dt / I.dx[t2];
     const long e = eidx(I, i, j, k);
     double *F2 = x.FT[pair_index(t1, t2)], *F3 = x.FT[pair_index(t2, t1)];
-    Face side[2];
+    FaceH side[2];
+    long l2[2], r2[2], r3[2];
+    double rho[2], rrnew[2];
+#pragma unroll
     for (int s = 0; s < 2; ++s) {
         const int sh = s - 1;
         const int il = i + sh * ed[0], jl = j + sh * ed[1], kl = k + sh * ed[2];
-        QIn lq;
-        load_qin_fs(s == 0 ? x.SM[d] : x.SP[d], I, e, lq);
-        double f2r[NVAR], f2l[NVAR], f3r[NVAR], f3l[NVAR], pg2p, ug2p, pg2m, ug2m, pg3p, ug3p, pg3m, ug3m;
-        load_tflux(F2, I, eidx(I, il + e1[0], jl + e1[1], kl + e1[2]), f2r, pg2p, ug2p);
-        load_tflux(F2, I, eidx(I, il, jl, kl), f2l, pg2m, ug2m);
-        load_tflux(F3, I, eidx(I, il + e2[0], jl + e2[1], kl + e2[2]), f3r, pg3p, ug3p);
-        load_tflux(F3, I, eidx(I, il, jl, kl), f3l, pg3m, ug3m);
+        l2[s] = eidx(I, il, jl, kl);
+        r2[s] = eidx(I, il + e1[0], jl + e1[1], kl + e1[2]);
+        r3[s] = eidx(I, il + e2[0], jl + e2[1], kl + e2[2]);
+        QInH lq;
+        load_qin_h(s == 0 ? x.SM[d] : x.SP[d], I, e, lq);
+        double f2r[UEDEN + 1], f2l[UEDEN + 1], f3r[UEDEN + 1], f3l[UEDEN + 1], pg2p, ug2p, pg2m, ug2m, pg3p, ug3p,
+            pg3m, ug3m;
+        load_tflux_h(F2, I, r2[s], f2r, pg2p, ug2p);
+        load_tflux_h(F2, I, l2[s], f2l, pg2m, ug2m);
+        load_tflux_h(F3, I, r3[s], f3r, pg3p, ug3p);
+        load_tflux_h(F3, I, l2[s], f3l, pg3m, ug3m);
         const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, il, jl, kl));
-        const QState o = trans2_state(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m, gamc,
-                                      h2, h3);
-        qstate_to_face(o, d, side[s]);
+        const QStateH o = trans2_hydro(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m, gamc,
+                                       h2, h3);
+        rho[s] = lq.rho;
+        rrnew[s] = lq.rho - h2 * (f2r[URHO] - f2l[URHO]) - h3 * (f3r[URHO] - f3l[URHO]);
+        side[s] = to_face(o, d);
     }
-    RiemannOut o;
-    face_riemann(x, I, d, i, j, k, side[0], side[1], o);
-    double f[NVAR];
-    const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
-    f[URHO] = o.f_rho;
-    f[mn] = o.f_mn;
-    f[mt] = o.f_mt;
-    f[mtt] = o.f_mtt;
-    f[UEDEN] = o.f_E;
-    f[UEINT] = o.f_e;
-    f[UTEMP] = 0.0;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) f[UFS + n] = o.f_rho * o.gx[n];
+    RiemannH o;
+    face_riemann_h(x, I, d, i, j, k, side[0], side[1], o);
 
     // artificial viscosity with the reference's corner-node pattern
     double div1;
@@ -517,22 +531,35 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     div1 = 0.1 * fmin(0.0, div1);
     const long cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
     const double dxd = I.dx[d];
+#define AVN(n, fval) ((fval) + dxd * (div1 * (CA(x.U1, I, NVAR, n, cr) - CA(x.U1, I, NVAR, n, cl))))
+    const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
+    double *FN = x.FN[d];
+    const double frho = AVN(URHO, o.f_rho);
+    EA(FN, I, NFN, URHO, e) = frho;
+    EA(FN, I, NFN, mn, e) = AVN(mn, o.f_mn);
+    EA(FN, I, NFN, mt, e) = AVN(mt, o.f_mt);
+    EA(FN, I, NFN, mtt, e) = AVN(mtt, o.f_mtt);
+    EA(FN, I, NFN, UEDEN, e) = AVN(UEDEN, o.f_E);
+    EA(FN, I, NFN, UEINT, e) = AVN(UEINT, o.f_e);
+    EA(FN, I, NFN, UTEMP, e) = 0.0;
+    double fs[NSPEC], sum = 0.0;
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) {
-        if (n == UTEMP) continue;
-        const double d1 = div1 * (CA(x.U1, I, NVAR, n, cr) - CA(x.U1, I, NVAR, n, cl));
-        f[n] = f[n] + dxd * d1;
+    for (int n = 0; n < NSPEC; ++n) {
+        const double x0 = trans2_species(rho[0], EA(x.SM[d], I, NFS, FS_X + n, e), EA(F2, I, NFF, FF_X + n, r2[0]),
+                                         EA(F2, I, NFF, FF_X + n, l2[0]), EA(F3, I, NFF, FF_X + n, r3[0]),
+                                         EA(F3, I, NFF, FF_X + n, l2[0]), h2, h3, rrnew[0]);
+        const double x1 = trans2_species(rho[1], EA(x.SP[d], I, NFS, FS_X + n, e), EA(F2, I, NFF, FF_X + n, r2[1]),
+                                         EA(F2, I, NFF, FF_X + n, l2[1]), EA(F3, I, NFF, FF_X + n, r3[1]),
+                                         EA(F3, I, NFF, FF_X + n, l2[1]), h2, h3, rrnew[1]);
+        fs[n] = AVN(UFS + n, o.f_rho * (o.fpl * x0 + o.fmr * x1));
+        sum = sum + fs[n];
     }
-    double sum = 0.0;
+#undef AVN
+    const double fac = (sum != 0.0) ? frho / sum : 1.0;
 #pragma unroll
-    for (int n = UFS; n < UFS + NSPEC; ++n) sum = sum + f[n];
-    const double fac = (sum != 0.0) ? f[URHO] / sum : 1.0;
-#pragma unroll
-    for (int n = UFS; n < UFS + NSPEC; ++n) f[n] = f[n] * fac;
-#pragma unroll
-    for (int n = 0; n < NVAR; ++n) EA(x.FN[d], I, NFN, n, e) = f[n];
-    EA(x.FN[d], I, NFN, FN_GP, e) = o.g_p;
-    EA(x.FN[d], I, NFN, FN_GU, e) = o.g_un;
+    for (int n = 0; n < NSPEC; ++n) EA(FN, I, NFN, UFS + n, e) = fs[n] * fac;
+    EA(FN, I, NFN, FN_GP, e) = o.g_p;
+    EA(FN, I, NFN, FN_GU, e) = o.g_un;
 }
 
 // fill_hydro_source (Source/Castro_hydro_nd.F90:344-441), Saxpy and the two

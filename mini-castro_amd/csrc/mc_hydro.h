@@ -155,25 +155,23 @@ __device__ __forceinline__ void ppm_project(const double I[3][4], double gam_ref
     o_p = fmax(small_pres, p_ref + (alphap + alpham) * csq_ref);
 }
 
-// Interface state in the order (rho, un, ut, utt, p, rhoe, X[13]) used by
-// the Riemann solver: directional velocities already permuted.
-struct Face {
+// Interface state without species, velocities permuted to the face direction
+// (un normal, ut / utt the two transverse components in the reference's order).
+struct FaceH {
     double rho, un, ut, utt, p, rhoe;
-    double x[NSPEC];
 };
 
-// Riemann solution at one face, Source/riemann.F90:114-325.
-// Outputs: flux (in conserved order, momenta directional: fl[UMX+0] =
-// normal, +1 = first transverse, +2 = second transverse), the Godunov
-// state (rho, un, ut, utt, p, game) and qint species.
-struct RiemannOut {
+// Riemann solution without species (Source/riemann.F90:114-318).  Species are
+// passive: their interface value is fpl*XL + fmr*XR (riemann.F90:285-288) and
+// their flux f_rho times that (:321-325).
+struct RiemannH {
     double f_rho, f_mn, f_mt, f_mtt, f_E, f_e;
     double g_rho, g_un, g_ut, g_utt, g_p, g_game, g_rhoe;
-    double gx[NSPEC];  // qint species; species flux = f_rho * gx
+    double fpl, fmr;
 };
 
-__device__ __forceinline__ void riemann_cgf(const Face &L, const Face &R, double cL, double cR, double gamcl,
-                                            double gamcr, RiemannOut &o)
+__device__ __forceinline__ void riemann_hydro(const FaceH &L, const FaceH &R, double cL, double cR, double gamcl,
+                                              double gamcr, RiemannH &o)
 {
     const double rl = fmax(L.rho, small_dens);
     const double ul = L.un, v1l = L.ut, v2l = L.utt;
@@ -258,6 +256,7 @@ __device__ __forceinline__ void riemann_cgf(const Face &L, const Face &R, double
     o.g_un = qun;
     o.g_p = qp;
     o.g_game = qgame;
+    o.g_rhoe = regdnv;
 
     const double u_adv = qun;
     const double rhoeint = regdnv;
@@ -268,38 +267,71 @@ __device__ __forceinline__ void riemann_cgf(const Face &L, const Face &R, double
     const double rhoetot = rhoeint + HALF * qrho * (qun * qun + o.g_ut * o.g_ut + o.g_utt * o.g_utt);
     o.f_E = u_adv * (rhoetot + qp);
     o.f_e = u_adv * rhoeint;
-    o.g_rhoe = regdnv;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) o.gx[n] = fpl * L.x[n] + fmr * R.x[n];
+    o.fpl = fpl;
+    o.fmr = fmr;
 }
 
-// Interface-state output of trans1/trans2 in primitive order.
-struct QState {
-    double rho, u, v, w, p, rhoe;
+// Interface state in the order (rho, un, ut, utt, p, rhoe, X[13]) used by
+// the Riemann solver: directional velocities already permuted.
+struct Face {
+    double rho, un, ut, utt, p, rhoe;
     double x[NSPEC];
-    double game;      // only defined when reset (copied through)
-    bool reset;
 };
+
+// Riemann solution at one face, Source/riemann.F90:114-325.
+struct RiemannOut {
+    double f_rho, f_mn, f_mt, f_mtt, f_E, f_e;
+    double g_rho, g_un, g_ut, g_utt, g_p, g_game, g_rhoe;
+    double gx[NSPEC];  // qint species; species flux = f_rho * gx
+};
+
+__device__ __forceinline__ void riemann_cgf(const Face &L, const Face &R, double cL, double cR, double gamcl,
+                                            double gamcr, RiemannOut &o)
+{
+    const FaceH lh = {L.rho, L.un, L.ut, L.utt, L.p, L.rhoe};
+    const FaceH rh = {R.rho, R.un, R.ut, R.utt, R.p, R.rhoe};
+    RiemannH h;
+    riemann_hydro(lh, rh, cL, cR, gamcl, gamcr, h);
+    o.f_rho = h.f_rho;
+    o.f_mn = h.f_mn;
+    o.f_mt = h.f_mt;
+    o.f_mtt = h.f_mtt;
+    o.f_E = h.f_E;
+    o.f_e = h.f_e;
+    o.g_rho = h.g_rho;
+    o.g_un = h.g_un;
+    o.g_ut = h.g_ut;
+    o.g_utt = h.g_utt;
+    o.g_p = h.g_p;
+    o.g_game = h.g_game;
+    o.g_rhoe = h.g_rhoe;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) o.gx[n] = h.fpl * L.x[n] + h.fmr * R.x[n];
+}
+
 
 // Interface state as stored in the qm/qp arrays (primitive order).
 struct QIn {
     double rho, u, v, w, p, rhoe, game;
     double x[NSPEC];
 };
+struct QInH {
+    double rho, u, v, w, p, rhoe, game;
+};
 
-// Source/trans.F90:151-229 for one state: lq2 corrected by the transverse
-// flux difference fr - fl (conserved components URHO..UEDEN and species)
-// and the Godunov p, u at the two faces.
-__device__ __forceinline__ QState trans1_state(const QIn &lq2, const double *fr, const double *fl, double pgp,
-                                               double pgm, double ugp, double ugm, double gamc, double cdtdx)
+// Interface-state output of a transverse update without species.
+struct QStateH {
+    double rho, u, v, w, p, rhoe;
+    double game;   // only defined when reset (copied through)
+    bool reset;
+};
+
+// Source/trans.F90:165-229 for one state: lq2 corrected by the transverse
+// flux difference fr - fl of URHO..UEDEN and the Godunov p, u at the faces.
+__device__ __forceinline__ QStateH trans1_hydro(const QInH &lq2, const double *fr, const double *fl, double pgp,
+                                                double pgm, double ugp, double ugm, double gamc, double cdtdx)
 {
-    QState o;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) {
-        const double rrnew = lq2.rho - cdtdx * (fr[URHO] - fl[URHO]);
-        const double compu = lq2.rho * lq2.x[n] - cdtdx * (fr[UFS + n] - fl[UFS + n]);
-        o.x[n] = compu / rrnew;
-    }
+    QStateH o;
     const double dup = pgp * ugp - pgm * ugm;
     const double du = ugp - ugm;
     const double pav = HALF * (pgp + pgm);
@@ -343,22 +375,23 @@ __device__ __forceinline__ QState trans1_state(const QIn &lq2, const double *fr,
     return o;
 }
 
-// Source/trans.F90:393-491 for one state: corrected by two transverse flux
-// differences (f2r - f2l in direction 2, f3r - f3l in direction 3).
-__device__ __forceinline__ QState trans2_state(const QIn &lq, const double *f2r, const double *f2l, const double *f3r,
-                                               const double *f3l, double pg2p, double pg2m, double ug2p, double ug2m,
-                                               double pg3p, double pg3m, double ug3p, double ug3m, double gamc,
-                                               double cdtdx2, double cdtdx3)
+// Source/trans.F90:151-158: one species of a trans1-corrected state.
+// rrnew = rho - cdtdx*(fr[URHO] - fl[URHO]), without the reset.
+__device__ __forceinline__ double trans1_species(double rho, double x, double frs, double fls, double cdtdx,
+                                                 double rrnew)
 {
-    QState o;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) {
-        const double rrr = lq.rho;
-        const double compr = rrr * lq.x[n];
-        const double rrnewr = rrr - cdtdx2 * (f2r[URHO] - f2l[URHO]) - cdtdx3 * (f3r[URHO] - f3l[URHO]);
-        const double compnr = compr - cdtdx2 * (f2r[UFS + n] - f2l[UFS + n]) - cdtdx3 * (f3r[UFS + n] - f3l[UFS + n]);
-        o.x[n] = compnr / rrnewr;
-    }
+    const double compu = rho * x - cdtdx * (frs - fls);
+    return compu / rrnew;
+}
+
+// Source/trans.F90:407-491 for one state without species: corrected by two
+// transverse flux differences (f2r - f2l in direction 2, f3r - f3l in 3).
+__device__ __forceinline__ QStateH trans2_hydro(const QInH &lq, const double *f2r, const double *f2l,
+                                                const double *f3r, const double *f3l, double pg2p, double pg2m,
+                                                double ug2p, double ug2m, double pg3p, double pg3m, double ug3p,
+                                                double ug3m, double gamc, double cdtdx2, double cdtdx3)
+{
+    QStateH o;
     const double du2p = pg2p * ug2p - pg2m * ug2m;
     const double p2av = HALF * (pg2p + pg2m);
     const double du2 = ug2p - ug2m;
@@ -404,6 +437,67 @@ __device__ __forceinline__ QState trans2_state(const QIn &lq, const double *f2r,
         o.game = lq.game;
     }
     o.p = fmax(pn, small_pres);
+    return o;
+}
+
+// Source/trans.F90:393-405: one species of a trans2-corrected state;
+// rrnew = rho - cdtdx2*(f2r[URHO]-f2l[URHO]) - cdtdx3*(f3r[URHO]-f3l[URHO]).
+__device__ __forceinline__ double trans2_species(double rho, double x, double f2rs, double f2ls, double f3rs,
+                                                 double f3ls, double cdtdx2, double cdtdx3, double rrnew)
+{
+    const double compr = rho * x;
+    const double compnr = compr - cdtdx2 * (f2rs - f2ls) - cdtdx3 * (f3rs - f3ls);
+    return compnr / rrnew;
+}
+
+// Interface-state output of trans1/trans2 in primitive order.
+struct QState {
+    double rho, u, v, w, p, rhoe;
+    double x[NSPEC];
+    double game;      // only defined when reset (copied through)
+    bool reset;
+};
+
+__device__ __forceinline__ QInH qin_hydro(const QIn &q) { return QInH{q.rho, q.u, q.v, q.w, q.p, q.rhoe, q.game}; }
+
+__device__ __forceinline__ void qstate_from(QState &o, const QStateH &h)
+{
+    o.rho = h.rho;
+    o.u = h.u;
+    o.v = h.v;
+    o.w = h.w;
+    o.p = h.p;
+    o.rhoe = h.rhoe;
+    o.game = h.game;
+    o.reset = h.reset;
+}
+
+// Source/trans.F90:151-229 for one state (hydro part plus all species).
+__device__ __forceinline__ QState trans1_state(const QIn &lq2, const double *fr, const double *fl, double pgp,
+                                               double pgm, double ugp, double ugm, double gamc, double cdtdx)
+{
+    QState o;
+    const double rrnew = lq2.rho - cdtdx * (fr[URHO] - fl[URHO]);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) o.x[n] = trans1_species(lq2.rho, lq2.x[n], fr[UFS + n], fl[UFS + n], cdtdx, rrnew);
+    qstate_from(o, trans1_hydro(qin_hydro(lq2), fr, fl, pgp, pgm, ugp, ugm, gamc, cdtdx));
+    return o;
+}
+
+// Source/trans.F90:393-491 for one state (hydro part plus all species).
+__device__ __forceinline__ QState trans2_state(const QIn &lq, const double *f2r, const double *f2l, const double *f3r,
+                                               const double *f3l, double pg2p, double pg2m, double ug2p, double ug2m,
+                                               double pg3p, double pg3m, double ug3p, double ug3m, double gamc,
+                                               double cdtdx2, double cdtdx3)
+{
+    QState o;
+    const double rrnew = lq.rho - cdtdx2 * (f2r[URHO] - f2l[URHO]) - cdtdx3 * (f3r[URHO] - f3l[URHO]);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n)
+        o.x[n] = trans2_species(lq.rho, lq.x[n], f2r[UFS + n], f2l[UFS + n], f3r[UFS + n], f3l[UFS + n], cdtdx2,
+                                cdtdx3, rrnew);
+    qstate_from(o, trans2_hydro(qin_hydro(lq), f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m,
+                                gamc, cdtdx2, cdtdx3));
     return o;
 }
 

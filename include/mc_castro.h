@@ -235,6 +235,27 @@ int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
 /* Launch one fused stage on the current device state (for timing). */
 int mc_level_run_stage(mc_level_t *L, int stage);
 
+/* ---- ensembles of independent single-box instances ----
+   Every instance is its own domain [0, n_cell) with its own problem constants
+   and time-step control; the fused engine advances all of them with one
+   launch per stage.  Instance state: 20 components, Fortran order over the
+   whole instance box (i fastest, component slowest). */
+typedef struct mc_ensemble mc_ensemble_t;
+/* n_cell, prob_lo, prob_hi: 3 per instance; bc: one per instance */
+mc_ensemble_t *mc_ensemble_create(int ninst, const int *n_cell, const double *prob_lo, const double *prob_hi,
+                                  const int *bc);
+void mc_ensemble_destroy(mc_ensemble_t *E);
+int mc_ensemble_ninst(const mc_ensemble_t *E);
+double *mc_ensemble_state_ptr(mc_ensemble_t *E, int m);
+/* problem constants: one value per instance */
+int mc_ensemble_init_data(mc_ensemble_t *E, const double *p_ambient, const double *dens_ambient,
+                          const double *exp_energy, const double *r_init, int nsub);
+int mc_ensemble_set_state(mc_ensemble_t *E, int m, const double *host);
+int mc_ensemble_get_state(mc_ensemble_t *E, int m, double *host);
+/* nsteps coarse steps of every instance, each with its own dt */
+int mc_ensemble_run(mc_ensemble_t *E, int nsteps, double stop_time, int use_graph);
+int mc_ensemble_clock(mc_ensemble_t *E, int m, double *dt, double *time, int *steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,15 @@ SIGS = [
     ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
     ("mc_level_run_stage", _I, [_vp, _I]),
     ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
+    ("mc_ensemble_create", _vp, [_I, _ip, _dp, _dp, _ip]),
+    ("mc_ensemble_destroy", None, [_vp]),
+    ("mc_ensemble_ninst", _I, [_vp]),
+    ("mc_ensemble_state_ptr", _dp, [_vp, _I]),
+    ("mc_ensemble_init_data", _I, [_vp, _dp, _dp, _dp, _dp, _I]),
+    ("mc_ensemble_set_state", _I, [_vp, _I, _dp]),
+    ("mc_ensemble_get_state", _I, [_vp, _I, _dp]),
+    ("mc_ensemble_run", _I, [_vp, _I, _D, _I]),
+    ("mc_ensemble_clock", _I, [_vp, _I, _dp, _dp, _ip]),
 ]
 
 _lib = None

@@ -77,3 +77,24 @@ def test_tiling_does_not_change_results():
     A.run(3, use_graph=False)
     B.run(3, use_graph=True)
     np.testing.assert_array_equal(A.state(), B.state())
+
+
+def test_ensemble_matches_sequential_runs():
+    """Instances of different sizes and constants advanced together, each with
+    its own dt, equal the same instances advanced one at a time by the per-box
+    path: every state bit and every instance's dt."""
+    insts = []
+    for side, E, r, rho, bc in ((16, 2e12, 0.12, 0.7, 1), (24, 5e12, 0.09, 1.6, 0), (20, 1e12, 0.15, 1.1, 1)):
+        d = small_cfg(n=side, bc=bc, r_init=r, exp_energy=E, dens_ambient=rho)
+        insts.append(d)
+    ens = mc.Ensemble(insts)
+    ens.init_data()
+    ens.run(2, use_graph=False)
+    ens.run(3, use_graph=True)
+    for m, d in enumerate(insts):
+        L = gpu_level(d, mode=mc.MODE_PLAIN)
+        for _ in range(5):
+            L.coarse_step()
+        dt, t, steps = ens.clock(m)
+        assert steps == 5 and dt == L.dt and t == L.time, (m, dt, L.dt)
+        np.testing.assert_array_equal(ens.state(m), L.state(), err_msg="instance %d" % m)

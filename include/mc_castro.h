@@ -225,6 +225,11 @@ int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph);
    (index 8: the time-step control kernels). */
 int mc_level_run_timed(mc_level_t *L, int nsteps, double stop_time, double *stage_ms, int *launches);
 int mc_level_steps(const mc_level_t *L);
+/* Castro::post_timestep's blast-radius diagnostic (Source/Castro.cpp:371-424):
+   max_density = S_new.max(Density); mass / radius sums of
+   calculate_blast_radius over every box; the printed radius is
+   (blast_radius / blast_mass) / 1e5 km. */
+int mc_level_blast_radius(mc_level_t *L, double *blast_mass, double *blast_radius, double *max_density);
 double mc_level_time(const mc_level_t *L);
 double mc_level_dt(const mc_level_t *L);
 int mc_level_sync(mc_level_t *L);

@@ -76,6 +76,36 @@ __global__ __launch_bounds__(kBlock) void k_saxpy(long n, double *s, const doubl
     if (t < n) s[t] = s[t] + dt * h[t];
 }
 
+// MultiFab::max of one component over a box arena: block max, then a
+// compare-and-swap max on *out (values are densities: finite, positive).
+__global__ __launch_bounds__(kBlock) void k_max_comp(long nvalid_cells_total, const double *arena, const long *cell0,
+                                                     const long *fab_off, const long *fab_n, int nbox, int comp,
+                                                     double *out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = -DBL_MAX;
+    if (t < nvalid_cells_total) {
+        int b = 0;
+        while (b + 1 < nbox && cell0[b + 1] <= t) ++b;
+        v = arena[fab_off[b] + fab_n[b] * comp + (t - cell0[b])];
+    }
+    __shared__ double red[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) v = fmax(v, red[w]);
+        unsigned long long *a = reinterpret_cast<unsigned long long *>(out);
+        unsigned long long old = *a;
+        while (v > __longlong_as_double(old)) {
+            const unsigned long long assumed = old;
+            old = atomicCAS(a, assumed, __double_as_longlong(v));
+            if (old == assumed) break;
+        }
+    }
+}
+
 __global__ void k_fill(long n, double *p, double v)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -642,6 +672,54 @@ double mc_level_coarse_step(mc_level_t *L, double stop_time)
     L->time += L->dt;
     L->steps += 1;
     return L->dt;
+}
+
+// Castro::post_timestep's diagnostic (Source/Castro.cpp:371-424): the
+// density maximum of S_new, then calculate_blast_radius over every box.
+int mc_level_blast_radius(mc_level_t *L, double *blast_mass, double *blast_radius, double *max_density)
+{
+    if (sync_from_fused(L)) return -1;
+    const int nbox = (int)L->boxes.size();
+    std::vector<long> cell0(nbox + 1), off(nbox), nn(nbox);
+    long acc = 0;
+    for (int b = 0; b < nbox; ++b) {
+        cell0[b] = acc;
+        nn[b] = box_npts(L->boxes[b]);
+        off[b] = L->off_valid[b];
+        acc += nn[b];
+    }
+    cell0[nbox] = acc;
+    long *dv = nullptr;
+    if (check_hip(hipMalloc(&dv, sizeof(long) * (3 * nbox + 1)), "hipMalloc(max)")) return -1;
+    std::vector<long> h(3 * nbox + 1);
+    for (int b = 0; b <= nbox; ++b) h[b] = cell0[b];
+    for (int b = 0; b < nbox; ++b) {
+        h[nbox + 1 + b] = off[b];
+        h[2 * nbox + 1 + b] = nn[b];
+    }
+    double mx = -DBL_MAX;
+    int rc = check_hip(hipMemcpyAsync(dv, h.data(), sizeof(long) * h.size(), hipMemcpyHostToDevice, L->stream), "h2d");
+    fill_const(L->stream, L->dt_dev, 1, -DBL_MAX);
+    if (!rc)
+        hipLaunchKernelGGL(k_max_comp, dim3(grid_for(acc)), dim3(kBlock), 0, L->stream, acc, L->S_new, dv,
+                           dv + nbox + 1, dv + 2 * nbox + 1, nbox, (int)URHO, L->dt_dev);
+    if (!rc) rc = check_hip(hipMemcpyAsync(&mx, L->dt_dev, sizeof(double), hipMemcpyDeviceToHost, L->stream), "d2h");
+    if (!rc) rc = check_hip(hipStreamSynchronize(L->stream), "max sync");
+    (void)hipFree(dv);
+    if (rc) return -1;
+    double m = 0.0, r = 0.0;
+    void *prev = mc_get_stream();
+    mc_set_stream(L->stream);
+    for (int b = 0; b < nbox; ++b) {
+        const Box &bx = L->boxes[b];
+        calculate_blast_radius(bx.lo, bx.hi, L->S_new + L->off_valid[b], bx.lo, bx.hi, L->dx, L->prob_lo,
+                               L->prob_hi, &m, &r, mx);
+    }
+    mc_set_stream(prev);
+    *blast_mass = m;
+    *blast_radius = r;
+    *max_density = mx;
+    return check_hip(hipGetLastError(), "blast radius");
 }
 
 int mc_level_steps(const mc_level_t *L) { return L->steps; }

@@ -83,6 +83,7 @@ SIGS = [
     ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
     ("mc_level_run_stage", _I, [_vp, _I]),
     ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
+    ("mc_level_blast_radius", _I, [_vp, _dp, _dp, _dp]),
     ("mc_ensemble_create", _vp, [_I, _ip, _dp, _dp, _ip]),
     ("mc_ensemble_destroy", None, [_vp]),
     ("mc_ensemble_ninst", _I, [_vp]),

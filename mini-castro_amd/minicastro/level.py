@@ -111,6 +111,12 @@ class Level:
     def run_stage(self, stage):
         check(self.L.mc_level_run_stage(self.h, int(stage)), "mc_level_run_stage")
 
+    def blast_radius(self):
+        """(blast_mass, blast_radius_sum, max_density) of post_timestep's diagnostic."""
+        m, r, x = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.mc_level_blast_radius(self.h, C.byref(m), C.byref(r), C.byref(x)), "mc_level_blast_radius")
+        return m.value, r.value, x.value
+
     def sync(self):
         check(self.L.mc_level_sync(self.h), "mc_level_sync")
 

@@ -49,6 +49,21 @@ STAGE_UNITS_PER_LAUNCH = {  # (stage) -> function of n_cell giving units per lau
 }
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_fused_stages_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            w = ln.split()
+            if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel and w[2] == workload:
+                return float(w[3])
+    return None
+
+
 def cpu_baseline(cfg_full, n=64, steps=2):
     """The oracle (CPU restatement, OpenMP over (1024,16,16) tiles) on an n^3
     single-box sample of the same problem; zone-updates/s."""
@@ -79,11 +94,25 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3, help="per-box comparison steps (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tiling-json", default=None,
+                    help="take the stage tiling from config.tiling of an earlier bench.py JSON line "
+                         "instead of running the autotuner")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU; this build does not run across GPUs (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import torch
+    if a.copy_calibration:
+        nbytes = a.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 5}))
+        return
     import minicastro as mc
     from minicastro import configs
 
@@ -95,6 +124,15 @@ def main():
     # batched fused path
     L = mc.Level(n, cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_FUSED)
     L.init_data(**cfg)
+    if a.tiling_json:
+        with open(a.tiling_json) as f:
+            prev = json.loads([ln for ln in f if ln.startswith("{")][-1])
+        for st, b in prev["config"]["tiling"].items():
+            L.set_tiling(L.STAGES.index(st), *b)
+        tuned = None
+    else:
+        tuned = L.autotune(reps=2)   # restores the state it times on
+    tiling = {L.STAGES[st]: list(L.get_tiling(st)) for st in range(1, 8)}
     L.run(a.warmup, use_graph=True)
     L.sync()
     torch.cuda.synchronize()
@@ -120,9 +158,11 @@ def main():
     avg_s = tot_ms / nl / 1e3
     traffic_alg = STAGE_BYTES[dom] * STAGE_UNITS_PER_LAUNCH[dom](n)
     achieved = traffic_alg / avg_s
+    traffic = measured_traffic("mc::K_" + dom, cfg["name"])
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "mc::K_" + dom,
-                "avg_launch_ms": tot_ms / nl, "algorithmic_bytes_per_launch": traffic_alg}
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "mc::K_" + dom,
+                "avg_launch_ms": tot_ms / nl, "algorithmic_bytes_per_launch": traffic_alg,
+                "traffic_GB_per_s": (traffic / avg_s / 1e9) if traffic else None}
 
     # per-box path (the reference's kernel order) on the same workload
     plain = None
@@ -145,6 +185,7 @@ def main():
            "config": {"workload": cfg["name"], "n_cell": list(n), "boxes": int(len(cfg["boxes"])),
                       "path": "batched fused, one launch per stage over all boxes",
                       "graph_replay_ms_per_step": graph_ms, "steps_advanced": fused_steps_done,
+                      "tiling": tiling, "tiling_source": "autotuner" if tuned else a.tiling_json,
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,
                       "stage_ms_per_step": {k: v[0] / a.steps for k, v in stages.items() if v[1]}},

@@ -239,6 +239,12 @@ int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
 /* Launch one fused stage on the current device state (for timing). */
 int mc_level_run_stage(mc_level_t *L, int stage);
+/* Fused mode: time every candidate brick of every stage of the step (reps
+   launches each) on the data of one coarse step and keep the fastest; the
+   state and the clock are restored.  bricks (8 x 3 ints) and best_ms (8)
+   receive the chosen tiling and its time per launch (stage 0, est, is not
+   tuned); either may be NULL. */
+int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms);
 
 /* ---- ensembles of independent single-box instances ----
    Every instance is its own domain [0, n_cell) with its own problem constants
@@ -260,6 +266,8 @@ int mc_ensemble_get_state(mc_ensemble_t *E, int m, double *host);
 /* nsteps coarse steps of every instance, each with its own dt */
 int mc_ensemble_run(mc_ensemble_t *E, int nsteps, double stop_time, int use_graph);
 int mc_ensemble_clock(mc_ensemble_t *E, int m, double *dt, double *time, int *steps);
+/* as mc_level_autotune, over every instance */
+int mc_ensemble_autotune(mc_ensemble_t *E, int reps, int *bricks, double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,11 @@ int mc_ensemble_run(mc_ensemble_t *E, int nsteps, double stop_time, int use_grap
     return check_hip(hipStreamSynchronize(E->stream), "ensemble run");
 }
 
+int mc_ensemble_autotune(mc_ensemble_t *E, int reps, int *bricks, double *best_ms)
+{
+    return fused_autotune(E->F, reps, E->stream, bricks, best_ms);
+}
+
 int mc_ensemble_clock(mc_ensemble_t *E, int m, double *dt, double *time, int *steps)
 {
     if (m < 0 || m >= E->ninst) return -1;

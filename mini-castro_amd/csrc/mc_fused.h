@@ -37,5 +37,6 @@ int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s);
 void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz);
 void fused_get_tiling(FusedLevel *F, int stage, int *bxyz);
 int fused_run_stage(FusedLevel *F, int stage, hipStream_t s);
+int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *best_ms);
 
 } // namespace mc

@@ -42,103 +42,11 @@
 #include "mc_common.h"
 #include "mc_eos.h"
 #include "mc_fused.h"
+#include "mc_fused_impl.h"
 #include "mc_hydro.h"
 #include "mc_internal.h"
 
 namespace mc {
-
-// primitive components of the fused path
-enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_X, FQ_C = FQ_X + NSPEC, FQ_G, NFQ };
-// traced / corrected interface states (velocities in absolute order)
-enum { FS_RHO = 0, FS_U, FS_V, FS_W, FS_P, FS_RE, FS_X, NFS = FS_X + NSPEC };
-// transverse-stage fluxes: conserved URHO..UEDEN, species, Godunov p and normal u
-enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_X, FF_GP = FF_X + NSPEC, FF_GU, NFF };
-// final fluxes: NVAR conserved components, Godunov p and normal u
-enum { FN_GP = NVAR, FN_GU, NFN };
-
-struct FInst {
-    int n[3];
-    int periodic;
-    double dx[3];
-    long offD, offE;  // cell offsets in the cell / extended index spaces
-    long nD, nE;      // cell counts
-    int ext[3];       // n + 2
-};
-
-// Brick tiling of a launch: blocks of bx*by*bz threads, one brick each.
-struct Plan {
-    const FInst *inst;
-    const int *blk0;  // first block of each instance, ninst+1 entries
-    int ninst;
-    int bx, by, bz;
-    int shift;        // first index: -shift
-    int add;          // extent: n + add (cells [0,n): 0,0; faces [0,n]: 0,1; extended [-1,n]: 1,2)
-};
-
-struct Ctx {
-    const FInst *inst;
-    double *U, *U1, *Q, *FL, *DV;
-    double *SM[3], *SP[3], *FD[3], *FT[6], *FN[3];
-    double *dt_cur, *dt_est, *time;
-    int *steps;
-    EosTable tab;
-};
-
-__device__ __forceinline__ int find_inst(const Plan &p)
-{
-    int lo = 0, hi = p.ninst - 1;
-    const int b = blockIdx.x;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (p.blk0[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Cell of this thread in the plan's range; false if outside.
-__device__ __forceinline__ bool plan_cell(const Plan &p, int &m, int &i, int &j, int &k)
-{
-    m = find_inst(p);
-    const FInst &I = p.inst[m];
-    const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.n[2] + p.add;
-    const int nb0 = (e0 + p.bx - 1) / p.bx, nb1 = (e1 + p.by - 1) / p.by;
-    const int b = blockIdx.x - p.blk0[m];
-    const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
-    const int t = threadIdx.x;
-    const int ti = t % p.bx, tj = (t / p.bx) % p.by, tk = t / (p.bx * p.by);
-    const int li = bi * p.bx + ti, lj = bj * p.by + tj, lk = bk * p.bz + tk;
-    i = li - p.shift;
-    j = lj - p.shift;
-    k = lk - p.shift;
-    return li < e0 && lj < e1 && lk < e2;
-}
-
-__device__ __forceinline__ int bmap(int i, int n, int periodic)
-{
-    return periodic ? ((i % n) + n) % n : min(max(i, 0), n - 1);
-}
-
-// index of a (boundary-mapped) cell in the cell space of instance I
-__device__ __forceinline__ long cidx(const FInst &I, int i, int j, int k)
-{
-    return (long)bmap(i, I.n[0], I.periodic) +
-           (long)I.n[0] * (bmap(j, I.n[1], I.periodic) + (long)I.n[1] * bmap(k, I.n[2], I.periodic));
-}
-// index of an extended cell / face (i,j,k in [-1, n])
-__device__ __forceinline__ long eidx(const FInst &I, int i, int j, int k)
-{
-    return (long)(i + 1) + (long)I.ext[0] * ((j + 1) + (long)I.ext[1] * (k + 1));
-}
-// component c of a cell array with C components
-__device__ __forceinline__ double &CA(double *a, const FInst &I, int C, int c, long idx)
-{
-    return a[(long)C * I.offD + (long)c * I.nD + idx];
-}
-__device__ __forceinline__ double &EA(double *a, const FInst &I, int C, int c, long idx)
-{
-    return a[(long)C * I.offE + (long)c * I.nE + idx];
-}
 
 // ------------------------------------------------------------------ stages
 // estdt on the current state (Castro::estTimeStep before the first step)
@@ -321,64 +229,6 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     }
 }
 
-// Hydro part (rho, velocities permuted to direction d, p, rhoe) of a state
-__device__ __forceinline__ void load_fs_h(double *S, const FInst &I, long e, int d, FaceH &f)
-{
-    const int un = FS_U + d, ut = d == 0 ? FS_V : FS_U, utt = d == 2 ? FS_V : FS_W;
-    f.rho = EA(S, I, NFS, FS_RHO, e);
-    f.un = EA(S, I, NFS, un, e);
-    f.ut = EA(S, I, NFS, ut, e);
-    f.utt = EA(S, I, NFS, utt, e);
-    f.p = EA(S, I, NFS, FS_P, e);
-    f.rhoe = EA(S, I, NFS, FS_RE, e);
-}
-
-__device__ __forceinline__ void load_qin_h(double *S, const FInst &I, long e, QInH &q)
-{
-    q.rho = EA(S, I, NFS, FS_RHO, e);
-    q.u = EA(S, I, NFS, FS_U, e);
-    q.v = EA(S, I, NFS, FS_V, e);
-    q.w = EA(S, I, NFS, FS_W, e);
-    q.p = EA(S, I, NFS, FS_P, e);
-    q.rhoe = EA(S, I, NFS, FS_RE, e);
-    q.game = 0.0;
-}
-
-__device__ __forceinline__ FaceH to_face(const QStateH &o, int d)
-{
-    FaceH f;
-    f.rho = o.rho;
-    f.un = d == 0 ? o.u : (d == 1 ? o.v : o.w);
-    f.ut = d == 0 ? o.v : o.u;
-    f.utt = d == 2 ? o.v : o.w;
-    f.p = o.p;
-    f.rhoe = o.rhoe;
-    return f;
-}
-
-// Non-species part of a transverse-stage flux; momenta in absolute order.
-__device__ __forceinline__ void store_tflux_h(double *F, const FInst &I, long e, int d, const RiemannH &o)
-{
-    const int mn = FF_MX + d, mt = d == 0 ? FF_MY : FF_MX, mtt = d == 2 ? FF_MY : FF_MZ;
-    EA(F, I, NFF, FF_RHO, e) = o.f_rho;
-    EA(F, I, NFF, mn, e) = o.f_mn;
-    EA(F, I, NFF, mt, e) = o.f_mt;
-    EA(F, I, NFF, mtt, e) = o.f_mtt;
-    EA(F, I, NFF, FF_E, e) = o.f_E;
-    EA(F, I, NFF, FF_GP, e) = o.g_p;
-    EA(F, I, NFF, FF_GU, e) = o.g_un;
-}
-
-// Riemann problem on face (i,j,k) of direction d (cells i-e_d and i)
-__device__ __forceinline__ void face_riemann_h(const Ctx &x, const FInst &I, int d, int i, int j, int k,
-                                               const FaceH &L, const FaceH &R, RiemannH &o)
-{
-    const long cr = cidx(I, i, j, k);
-    const long cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
-    riemann_hydro(L, R, CA(x.Q, I, NFQ, FQ_C, cl), CA(x.Q, I, NFQ, FQ_C, cr), CA(x.Q, I, NFQ, FQ_G, cl),
-                  CA(x.Q, I, NFQ, FQ_G, cr), o);
-}
-
 // Source/riemann.F90:10-331 on the PPM states: F^d on d-faces
 // (d-face index 0..n_d, transverse cells -1..n).  Species are streamed.
 __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
@@ -403,18 +253,6 @@ __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
         EA(F, I, NFF, FF_X + n, e) = o.f_rho * gx;
     }
 }
-
-// URHO..UEDEN of a transverse-stage flux and its Godunov p, u
-__device__ __forceinline__ void load_tflux_h(double *F, const FInst &I, long e, double *v, double &gp, double &gu)
-{
-#pragma unroll
-    for (int n = 0; n <= UEDEN; ++n) v[n] = EA(F, I, NFF, FF_RHO + n, e);
-    gp = EA(F, I, NFF, FF_GP, e);
-    gu = EA(F, I, NFF, FF_GU, e);
-}
-
-__constant__ int c_pair_t[6] = {1, 2, 0, 2, 0, 1};   // state (face) direction
-__constant__ int c_pair_d[6] = {0, 0, 1, 1, 2, 2};   // flux direction
 
 // trans1 (Source/trans.F90:14-243) of the t-face states by the d-flux, then
 // the flux F^{t|d} of the corrected states on the t-faces of tbx[t][d].
@@ -464,13 +302,6 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
                                          EA(FDd, I, NFF, FF_X + n, el[1]), cdt, rrnew[1]);
         EA(F, I, NFF, FF_X + n, e) = o.f_rho * (o.fpl * x0 + o.fmr * x1);
     }
-}
-
-// index of the F^{a|b} array
-__device__ __forceinline__ int pair_index(int a, int b)
-{
-    // combos: (1,0) (2,0) (0,1) (2,1) (0,2) (1,2)
-    return 2 * b + ((a == 2) || (b == 2 && a == 1) ? 1 : 0);
 }
 
 // trans2 (Source/trans.F90:250-505) + final flux + apply_av
@@ -646,11 +477,6 @@ __global__ void K_fill(double *p, long n, double v)
     if (t < n) p[t] = v;
 }
 
-// box arena <-> instance-0 cell layout
-struct ArenaBox {
-    int lo[3], hi[3];
-    long off;
-};
 __global__ __launch_bounds__(256) void K_arena(const ArenaBox *boxes, int nbox, const long *bstart, double *arena,
                                                double *U, long nD, int n0, int n1, int to_arena)
 {
@@ -673,44 +499,11 @@ __global__ __launch_bounds__(256) void K_arena(const ArenaBox *boxes, int nbox, 
 }
 
 // ------------------------------------------------------------------ host side
-struct KernelTiling {
-    int bx, by, bz;
-};
-
-struct FusedLevel {
-    std::vector<FInst> hinst;
-    int ninst = 0;
-    FInst *inst = nullptr;
-    long totD = 0, totE = 0;
-    Ctx ctx;
-    double *arena_buf = nullptr;
-    // plans: one block table per (tiling, ghost) pair
-    struct PlanBuf {
-        KernelTiling t;
-        int shift, add;
-        int *blk0 = nullptr;
-        int nblocks = 0;
-    };
-    std::vector<PlanBuf> plans;
-    KernelTiling tiling[8];  // per stage: est prim flat trace flux1 trans1 final update
-    bool have_est = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    double graph_stop = -2.0;
-    // single-level import/export
-    ArenaBox *abox = nullptr;
-    long *abstart = nullptr;
-    int nabox = 0;
-    long arena_n = 0;
-};
-
-enum { ST_EST = FST_EST, ST_PRIM = FST_PRIM, ST_FLAT = FST_FLAT, ST_TRACE = FST_TRACE, ST_FLUX1 = FST_FLUX1,
-       ST_TRANS1 = FST_TRANS1, ST_FINAL = FST_FINAL, ST_UPDATE = FST_UPDATE, NSTAGE = FST_N };
 // index range of each stage: cells [0,n), faces [0,n], or the ring [-1,n]
 static const int stage_shift[NSTAGE] = {0, 0, 1, 1, 1, 1, 0, 0};
 static const int stage_add[NSTAGE] = {0, 0, 2, 2, 2, 2, 1, 0};
 
-static Plan get_plan(FusedLevel *F, int stage)
+Plan get_plan(FusedLevel *F, int stage)
 {
     const KernelTiling t = F->tiling[stage];
     const int sh = stage_shift[stage], add = stage_add[stage];
@@ -738,7 +531,7 @@ static Plan get_plan(FusedLevel *F, int stage)
     return Plan{F->inst, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
 }
 
-static int plan_blocks(FusedLevel *F, const Plan &p)
+int plan_blocks(FusedLevel *F, const Plan &p)
 {
     for (auto &pb : F->plans)
         if (pb.blk0 == p.blk0) return pb.nblocks;
@@ -771,7 +564,7 @@ static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1)
     }
 }
 
-static void fill(double *p, long n, double v, hipStream_t s)
+void fill(double *p, long n, double v, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(K_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
@@ -1083,6 +876,95 @@ int fused_run_stage(FusedLevel *F, int stage, hipStream_t s)
     static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
     launch_stage(F, stage, s, ny[stage]);
     return check_hip(hipGetLastError(), "fused_run_stage");
+}
+
+// Brick candidates of the tiling autotuner: 64, 128 and 256 threads, from
+// i-pencils (64,1,1) to cubes (8,8,4).
+static const KernelTiling tune_cand[] = {
+    {64, 1, 1}, {32, 2, 1}, {16, 4, 1}, {8, 8, 1},  {16, 2, 2}, {128, 1, 1}, {64, 2, 1}, {32, 4, 1},
+    {16, 8, 1}, {32, 2, 2}, {16, 4, 2}, {8, 8, 2},  {64, 4, 1}, {64, 2, 2},  {32, 8, 1}, {32, 4, 2},
+    {16, 16, 1}, {16, 8, 2}, {16, 4, 4}, {8, 8, 4}, {128, 2, 1}};
+
+// Tiling autotuner: times every candidate brick of every stage of the step
+// on the data one coarse step leaves behind and keeps the fastest.  No stage
+// reads an array it writes, so repeating a launch recomputes the same
+// outputs; the state and the clocks are restored afterwards, so results do
+// not change (tilings never change results).  bricks (8 x 3) and best_ms (8)
+// receive the chosen brick and its time per launch; either may be null.
+int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *best_ms)
+{
+    if (reps < 1) reps = 1;
+    if (!F->have_est && fused_estimate(F, s)) return -1;
+    const int ni = F->ninst;
+    double *saveU = nullptr, *saveC = nullptr;
+    int *saveS = nullptr;
+    const size_t nU = sizeof(double) * NVAR * (size_t)F->totD;
+    if (check_hip(hipMalloc(&saveU, nU), "hipMalloc(autotune)") ||
+        check_hip(hipMalloc(&saveC, sizeof(double) * 3 * ni), "hipMalloc(autotune)") ||
+        check_hip(hipMalloc(&saveS, sizeof(int) * ni), "hipMalloc(autotune)")) {
+        (void)hipFree(saveU);
+        (void)hipFree(saveC);
+        return -1;
+    }
+    Ctx &x = F->ctx;
+    int rc = check_hip(hipMemcpyAsync(saveU, x.U, nU, hipMemcpyDeviceToDevice, s), "save") ||
+             check_hip(hipMemcpyAsync(saveC, x.dt_cur, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "save") ||
+             check_hip(hipMemcpyAsync(saveC + ni, x.dt_est, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "save") ||
+             check_hip(hipMemcpyAsync(saveC + 2 * ni, x.time, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "save") ||
+             check_hip(hipMemcpyAsync(saveS, x.steps, sizeof(int) * ni, hipMemcpyDeviceToDevice, s), "save");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = check_hip(hipEventCreate(&e0), "hipEventCreate") || check_hip(hipEventCreate(&e1), "hipEventCreate");
+    if (!rc) enqueue_step(F, -1.0, s);  // every intermediate array now holds data of a real step
+    static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    for (int st = ST_PRIM; st <= ST_UPDATE && !rc; ++st) {
+        KernelTiling best = F->tiling[st];
+        double tbest = 1e300;
+        for (const KernelTiling &c : tune_cand) {
+            F->tiling[st] = c;
+            launch_stage(F, st, s, ny[st]);  // creates the block table outside the timed region
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; ++r) launch_stage(F, st, s, ny[st]);
+            (void)hipEventRecord(e1, s);
+            float ms = 0.f;
+            if ((rc = check_hip(hipEventSynchronize(e1), "autotune sync") ||
+                      check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")))
+                break;
+            if (ms / reps < tbest) {
+                tbest = ms / reps;
+                best = c;
+            }
+        }
+        F->tiling[st] = best;
+        if (bricks) {
+            bricks[3 * st] = best.bx;
+            bricks[3 * st + 1] = best.by;
+            bricks[3 * st + 2] = best.bz;
+        }
+        if (best_ms) best_ms[st] = tbest;
+    }
+    if (bricks) {
+        bricks[0] = F->tiling[ST_EST].bx;
+        bricks[1] = F->tiling[ST_EST].by;
+        bricks[2] = F->tiling[ST_EST].bz;
+    }
+    if (best_ms) best_ms[ST_EST] = 0.0;
+    // restore the state and the clocks
+    rc = check_hip(hipMemcpyAsync(x.U, saveU, nU, hipMemcpyDeviceToDevice, s), "restore") ||
+         check_hip(hipMemcpyAsync(x.dt_cur, saveC, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
+         check_hip(hipMemcpyAsync(x.dt_est, saveC + ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
+         check_hip(hipMemcpyAsync(x.time, saveC + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
+         check_hip(hipMemcpyAsync(x.steps, saveS, sizeof(int) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
+         check_hip(hipStreamSynchronize(s), "autotune restore") || rc;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(saveU);
+    (void)hipFree(saveC);
+    (void)hipFree(saveS);
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    F->gexec = nullptr;
+    F->graph = nullptr;
+    return rc ? -1 : 0;
 }
 
 } // namespace mc

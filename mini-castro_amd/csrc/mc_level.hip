@@ -394,6 +394,20 @@ int mc_level_run_stage(mc_level_t *L, int stage)
     return fused_run_stage(L->fused, stage, L->stream);
 }
 
+int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms)
+{
+    if (L->mode != MC_MODE_FUSED) {
+        set_error("mc_level_autotune: fused mode only");
+        return -1;
+    }
+    if (!eos_ready()) {
+        set_error("mc_level_autotune: EOS table not loaded");
+        return -1;
+    }
+    if (!L->fused_ahead && sync_to_fused(L)) return -1;
+    return fused_autotune(L->fused, reps, L->stream, bricks, best_ms);
+}
+
 int mc_level_nbox(const mc_level_t *L) { return (int)L->boxes.size(); }
 
 double *mc_level_state_ptr(mc_level_t *L, int b)

@@ -82,6 +82,7 @@ SIGS = [
     ("mc_level_set_tiling", _I, [_vp, _I, _I, _I, _I]),
     ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
     ("mc_level_run_stage", _I, [_vp, _I]),
+    ("mc_level_autotune", _I, [_vp, _I, _ip, _dp]),
     ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
     ("mc_level_blast_radius", _I, [_vp, _dp, _dp, _dp]),
     ("mc_ensemble_create", _vp, [_I, _ip, _dp, _dp, _ip]),
@@ -93,6 +94,7 @@ SIGS = [
     ("mc_ensemble_get_state", _I, [_vp, _I, _dp]),
     ("mc_ensemble_run", _I, [_vp, _I, _D, _I]),
     ("mc_ensemble_clock", _I, [_vp, _I, _dp, _dp, _ip]),
+    ("mc_ensemble_autotune", _I, [_vp, _I, _ip, _dp]),
 ]
 
 _lib = None

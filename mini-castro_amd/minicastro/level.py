@@ -111,6 +111,14 @@ class Level:
     def run_stage(self, stage):
         check(self.L.mc_level_run_stage(self.h, int(stage)), "mc_level_run_stage")
 
+    def autotune(self, reps=3):
+        """Fused mode: pick the fastest brick of every stage (state unchanged).
+        Returns {stage: ((bx, by, bz), ms_per_launch)}."""
+        b = (C.c_int * 24)()
+        ms = (C.c_double * 8)()
+        check(self.L.mc_level_autotune(self.h, int(reps), b, ms), "mc_level_autotune")
+        return {self.STAGES[s]: ((b[3 * s], b[3 * s + 1], b[3 * s + 2]), ms[s]) for s in range(1, 8)}
+
     def blast_radius(self):
         """(blast_mass, blast_radius_sum, max_density) of post_timestep's diagnostic."""
         m, r, x = C.c_double(), C.c_double(), C.c_double()

@@ -79,6 +79,30 @@ def test_tiling_does_not_change_results():
     np.testing.assert_array_equal(A.state(), B.state())
 
 
+def test_autotune_keeps_state_and_results():
+    """The autotuner leaves state and clock untouched, picks valid bricks, and
+    the tuned level advances bit-identically to an untuned one."""
+    cfg = small_cfg(n=32, bc=1, r_init=0.1)
+    boxes = configs.uneven_boxes(32, seed=5, parts=2, sizes=(8, 16, 24))
+    A = gpu_level(cfg, boxes=boxes)
+    B = gpu_level(cfg, boxes=boxes)
+    A.run(1, use_graph=False)
+    B.run(1, use_graph=False)
+    s0 = A.state()
+    clock0 = (A.steps, A.dt, A.time)
+    tuned = A.autotune(reps=1)
+    assert set(tuned) == {"prim", "flat", "trace", "flux1", "trans1", "final", "update"}
+    for st, ((bx, by, bz), ms) in tuned.items():
+        assert (bx * by * bz) % 64 == 0 and bx * by * bz <= 256 and ms > 0.0, (st, bx, by, bz, ms)
+        assert A.get_tiling(A.STAGES.index(st)) == (bx, by, bz)
+    np.testing.assert_array_equal(A.state(), s0)
+    assert (A.steps, A.dt, A.time) == clock0
+    A.run(3, use_graph=True)
+    B.run(3, use_graph=True)
+    assert (A.steps, A.dt, A.time) == (B.steps, B.dt, B.time)
+    np.testing.assert_array_equal(A.state(), B.state())
+
+
 def test_ensemble_matches_sequential_runs():
     """Instances of different sizes and constants advanced together, each with
     its own dt, equal the same instances advanced one at a time by the per-box

@@ -86,6 +86,50 @@ def cpu_baseline(cfg_full, n=64, steps=2):
                       "oracle OpenMP over (1024,16,16) tiles" % (n, steps)}
 
 
+def adjoint_bench(reps=3):
+    """Discrete adjoint on the 64^3 Sedov adjoint workload, 10 steps: time of
+    the checkpointing forward pass and of the adjoint pass (recompute + the
+    reversed stages), and the central-difference check of two smooth random
+    directions (eps 1e-6 and 1e-7)."""
+    import numpy as np
+    import torch
+    from minicastro import configs
+    from adjoint_harness import SedovAdjointProblem, smooth_direction
+    cfg = configs.adjoint_problem(n=64, steps=10)
+    P = SedovAdjointProblem(cfg)
+    J, g_rho, g_E = P.gradient()
+    U = P.U0(P.rho0, torch.tensor(P.E, dtype=torch.float64, device="cuda")).contiguous()
+    out, gin = torch.empty_like(U), torch.empty_like(U)
+    seed = torch.zeros_like(U)
+    seed[0] = 2.0 * P.mask
+    fwd, adj = [], []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.adj.forward(U.data_ptr(), out.data_ptr(), cfg["steps"], P.dts)
+        t1 = time.perf_counter()
+        P.adj.backward(seed.data_ptr(), gin.data_ptr())
+        t2 = time.perf_counter()
+        if r:
+            fwd.append(t1 - t0)
+            adj.append(t2 - t1)
+    rng = np.random.default_rng(7)
+    errs = []
+    for q in range(2):
+        v = P.rho0 * torch.from_numpy(smooth_direction(rng, 64)).to("cuda")
+        vE = P.E * rng.standard_normal()
+        ad = float((g_rho * v).sum()) + g_E * vE
+        e = []
+        for eps in (1e-6, 1e-7):
+            fd = (P.J(P.rho0 + eps * v, P.E + eps * vE) - P.J(P.rho0 - eps * v, P.E - eps * vE)) / (2 * eps)
+            e.append(abs(fd - ad) / abs(ad))
+        errs.append(e)
+    f, a_ = float(np.mean(fwd)) * 1e3, float(np.mean(adj)) * 1e3
+    return {"workload": cfg["name"], "steps": cfg["steps"], "forward_ms": f, "adjoint_ms": a_,
+            "adjoint_over_forward": a_ / f, "J": J, "dJ_dE": g_E,
+            "fd_rel_err_smooth_dirs": {"eps_1e-6": [e[0] for e in errs], "eps_1e-7": [e[1] for e in errs]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +138,7 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3, help="per-box comparison steps (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--tiling-json", default=None,
                     help="take the stage tiling from config.tiling of an earlier bench.py JSON line "
                          "instead of running the autotuner")
@@ -178,6 +223,7 @@ def main():
         plain = (p1 - p0) * 1e3 / a.plain_steps
         P.close()
 
+    adjoint = None if a.no_adjoint else adjoint_bench()
     cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
     out = {"metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -189,7 +235,7 @@ def main():
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,
                       "stage_ms_per_step": {k: v[0] / a.steps for k, v in stages.items() if v[1]}},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint}
     print(json.dumps(out))
 
 

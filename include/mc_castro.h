@@ -269,6 +269,28 @@ int mc_ensemble_clock(mc_ensemble_t *E, int m, double *dt, double *time, int *st
 /* as mc_level_autotune, over every instance */
 int mc_ensemble_autotune(mc_ensemble_t *E, int reps, int *bricks, double *best_ms);
 
+/* ---- discrete adjoint of the CTU step (one single-box instance) ----
+   Forward: U0 -> U_N over nsteps steps of the fused path with a frozen dt
+   sequence, checkpointing the state at the start of every step.  Backward:
+   given the adjoint of U_N, recompute each step from its checkpoint and
+   apply the hand-written reverse of every stage, returning the adjoint of
+   U0.  The differentiated components are rho, rho u, rho v, rho w, rho E
+   and rho e; UTEMP (only the EOS Newton initial guess) and the species
+   (whose influence through abar/zbar after normalize_species is of the
+   order of its 1e-30 floor) are held constant.  All state arrays are
+   device pointers, NVAR x cells in Fortran order; calls run on the stream
+   given to mc_set_stream and return after it has drained. */
+typedef struct mc_adjoint mc_adjoint_t;
+mc_adjoint_t *mc_adjoint_create(const int *n_cell, const double *prob_lo, const double *prob_hi, int bc,
+                                int max_steps);
+void mc_adjoint_destroy(mc_adjoint_t *A);
+/* compute_dt != 0: dts[0..nsteps-1] receive dt_0 = cfl * estdt(U0) and
+   dt_{n+1} = min(cfl * estdt(U_{n+1}), 1.1 dt_n); otherwise dts are used. */
+int mc_adjoint_forward(mc_adjoint_t *A, const double *U0, double *U_out, int nsteps, double *dts, int compute_dt);
+/* Ub0 = (dU_N/dU0)^T Ub_final for the last forward pass */
+int mc_adjoint_backward(mc_adjoint_t *A, const double *Ub_final, double *Ub0);
+int mc_adjoint_steps(const mc_adjoint_t *A);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,7 @@ int fused_estimate(FusedLevel *F, hipStream_t s);
 int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool use_graph);
 int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, double *stage_ms, int *launches);
 int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s);
+int fused_recompute_dt(FusedLevel *F, double dt, hipStream_t s);
 int fused_set_clock(FusedLevel *F, int m, double dt, double time, int steps, hipStream_t s);
 int fused_read_clock(FusedLevel *F, int m, double *dt, double *time, int *steps, hipStream_t s);
 int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s);

@@ -825,6 +825,20 @@ int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s)
     return check_hip(hipGetLastError(), "fused_advance_dt");
 }
 
+// The stages of a step up to the final fluxes with a given dt (the state is
+// not updated): recomputes every intermediate array of the step.
+int fused_recompute_dt(FusedLevel *F, double dt, hipStream_t s)
+{
+    fill(F->ctx.dt_cur, F->ninst, dt, s);
+    launch_stage(F, ST_PRIM, s);
+    launch_stage(F, ST_FLAT, s);
+    launch_stage(F, ST_TRACE, s, 3);
+    launch_stage(F, ST_FLUX1, s, 3);
+    launch_stage(F, ST_TRANS1, s, 6);
+    launch_stage(F, ST_FINAL, s, 3);
+    return check_hip(hipGetLastError(), "fused_recompute_dt");
+}
+
 __global__ void K_setclock(Ctx x, int m, double dt, double time, int steps)
 {
     x.dt_cur[m] = dt;

@@ -95,6 +95,11 @@ SIGS = [
     ("mc_ensemble_run", _I, [_vp, _I, _D, _I]),
     ("mc_ensemble_clock", _I, [_vp, _I, _dp, _dp, _ip]),
     ("mc_ensemble_autotune", _I, [_vp, _I, _ip, _dp]),
+    ("mc_adjoint_create", _vp, [_ip, _dp, _dp, _I, _I]),
+    ("mc_adjoint_destroy", None, [_vp]),
+    ("mc_adjoint_forward", _I, [_vp, _vp, _vp, _I, _dp, _I]),
+    ("mc_adjoint_backward", _I, [_vp, _vp, _vp]),
+    ("mc_adjoint_steps", _I, [_vp]),
 ]
 
 _lib = None

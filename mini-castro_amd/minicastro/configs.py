@@ -70,6 +70,23 @@ def config(idx):
     raise ValueError("no configuration %r" % idx)
 
 
+# Adjoint workload: 64^3 single box, 10 steps.  The deposition radius and
+# energy are chosen so that the blast front reaches the objective's shell
+# (r in [0.2, 0.3]) within the 10 CFL-limited steps: it starts at r = 0.15
+# (9.6 cells) and moves about half a cell per step.
+ADJOINT_SEDOV = dict(UNIT_SEDOV, r_init=0.15, exp_energy=1.e14)
+
+
+def adjoint_problem(n=64, steps=10, seed=11):
+    """Adjoint workload: Sedov on n^3 cells in one box, initial density
+    rho_amb (1 + 0.01 N(0,1)) seeded, objective sum(rho^2) over cells whose
+    centres lie at radius [0.2, 0.3] from the domain centre."""
+    d = dict(ADJOINT_SEDOV)
+    d.update(name="adjoint_sedov%d" % n, n_cell=(n,) * 3, boxes=grid_boxes(n, n), steps=steps,
+             noise=0.01, seed=seed, shell=(0.2, 0.3))
+    return d
+
+
 def ensemble(n_inst=24, seed=7):
     """Independent single-box Sedov instances (side, E, r_dep, rho_amb seeded)."""
     rng = np.random.default_rng(seed)

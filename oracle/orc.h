@@ -20,6 +20,15 @@
 #ifndef ORC_H
 #define ORC_H
 
+/* Scalar type of the restated arithmetic: double; the tangent-linear build
+   (compiled as C++ with ORC_TL) substitutes a forward-mode dual number. */
+#ifdef ORC_TL
+#include "orc_dual.h"
+typedef orc_dual real;
+#else
+typedef double real;
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -34,10 +43,10 @@ enum { EOS_INPUT_RT = 1, EOS_INPUT_RE = 2, EOS_INPUT_RP = 3 };
 /* eos_t, Source/eos.F90:18-58, plus the last table indices and the Newton
    iteration count (diagnostics the reference does not expose). */
 typedef struct {
-    double rho, T, p, e, h, s, abar, zbar;
-    double dpdT, dpdr, dedT, dedr, dhdT, dhdr, dsdT, dsdr, dpde, dpdr_e;
-    double cv, cp, xne, xnp, eta, pele, ppos, mu, mu_e, y_e, gam1, cs;
-    double dpdA, dpdZ, dedA, dedZ;
+    real rho, T, p, e, h, s, abar, zbar;
+    real dpdT, dpdr, dedT, dedr, dhdT, dhdr, dsdT, dsdr, dpde, dpdr_e;
+    real cv, cp, xne, xnp, eta, pele, ppos, mu, mu_e, y_e, gam1, cs;
+    real dpdA, dpdZ, dedA, dedZ;
     int iat, jat, niter, pad_;
 } orc_eos_t;
 
@@ -50,119 +59,119 @@ void orc_eos_grid(double *t201, double *d541); /* the temperature / density grid
 
 /* Per-box kernels, same argument lists as Source/Castro_F.H */
 void orc_ctoprim(const int *lo, const int *hi,
-                 const double *u, const int *u_lo, const int *u_hi,
-                 double *q, const int *q_lo, const int *q_hi,
-                 double *qaux, const int *qa_lo, const int *qa_hi);
+                 const real *u, const int *u_lo, const int *u_hi,
+                 real *q, const int *q_lo, const int *q_hi,
+                 real *qaux, const int *qa_lo, const int *qa_hi);
 void orc_divu(const int *lo, const int *hi,
-              const double *q, const int *q_lo, const int *q_hi,
-              const double *dx, double *div, const int *d_lo, const int *d_hi);
-void orc_uflatten(int i, int j, int k, const double *q, const int *q_lo, const int *q_hi,
-                  double *flatn);
-void orc_ppm_reconstruct(const double *s5, double flatn, double *sm, double *sp);
-void orc_ppm_int_profile(double sm, double sp, double sc, double u, double c, double dtdx,
-                         double *Ip3, double *Im3);
+              const real *q, const int *q_lo, const int *q_hi,
+              const real *dx, real *div, const int *d_lo, const int *d_hi);
+void orc_uflatten(int i, int j, int k, const real *q, const int *q_lo, const int *q_hi,
+                  real *flatn);
+void orc_ppm_reconstruct(const real *s5, real flatn, real *sm, real *sp);
+void orc_ppm_int_profile(real sm, real sp, real sc, real u, real c, real dtdx,
+                         real *Ip3, real *Im3);
 void orc_trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi, int idir,
-                   const double *q, const int *q_lo, const int *q_hi,
-                   const double *qaux, const int *qa_lo, const int *qa_hi,
-                   double *qm, const int *qm_lo, const int *qm_hi,
-                   double *qp, const int *qp_lo, const int *qp_hi,
-                   const int *domlo, const int *domhi, const double *dx, double dt);
+                   const real *q, const int *q_lo, const int *q_hi,
+                   const real *qaux, const int *qa_lo, const int *qa_hi,
+                   real *qm, const int *qm_lo, const int *qm_hi,
+                   real *qp, const int *qp_lo, const int *qp_hi,
+                   const int *domlo, const int *domhi, const real *dx, real dt);
 void orc_compute_flux(const int *lo, const int *hi,
-                      const double *ql, const int *ql_lo, const int *ql_hi,
-                      const double *qr, const int *qr_lo, const int *qr_hi,
-                      double *flx, const int *f_lo, const int *f_hi,
-                      double *qint, const int *qi_lo, const int *qi_hi,
-                      double *qgdnv, const int *qg_lo, const int *qg_hi,
-                      const double *qaux, const int *qa_lo, const int *qa_hi, int idir);
+                      const real *ql, const int *ql_lo, const int *ql_hi,
+                      const real *qr, const int *qr_lo, const int *qr_hi,
+                      real *flx, const int *f_lo, const int *f_hi,
+                      real *qint, const int *qi_lo, const int *qi_hi,
+                      real *qgdnv, const int *qg_lo, const int *qg_hi,
+                      const real *qaux, const int *qa_lo, const int *qa_hi, int idir);
 void orc_trans1(const int *lo, const int *hi, int idir1, int idir2,
-                const double *q2m, const int *q2m_lo, const int *q2m_hi,
-                double *q2mo, const int *q2mo_lo, const int *q2mo_hi,
-                const double *q2p, const int *q2p_lo, const int *q2p_hi,
-                double *q2po, const int *q2po_lo, const int *q2po_hi,
-                const double *qaux, const int *qa_lo, const int *qa_hi,
-                const double *f1, const int *f1_lo, const int *f1_hi,
-                const double *q1, const int *q1_lo, const int *q1_hi, double cdtdx);
+                const real *q2m, const int *q2m_lo, const int *q2m_hi,
+                real *q2mo, const int *q2mo_lo, const int *q2mo_hi,
+                const real *q2p, const int *q2p_lo, const int *q2p_hi,
+                real *q2po, const int *q2po_lo, const int *q2po_hi,
+                const real *qaux, const int *qa_lo, const int *qa_hi,
+                const real *f1, const int *f1_lo, const int *f1_hi,
+                const real *q1, const int *q1_lo, const int *q1_hi, real cdtdx);
 void orc_trans2(const int *lo, const int *hi, int idir1, int idir2, int idir3,
-                const double *qm, const int *qm_lo, const int *qm_hi,
-                double *qmo, const int *qmo_lo, const int *qmo_hi,
-                const double *qp, const int *qp_lo, const int *qp_hi,
-                double *qpo, const int *qpo_lo, const int *qpo_hi,
-                const double *qaux, const int *qa_lo, const int *qa_hi,
-                const double *f2, const int *f2_lo, const int *f2_hi,
-                const double *f3, const int *f3_lo, const int *f3_hi,
-                const double *q2, const int *q2_lo, const int *q2_hi,
-                const double *q3, const int *q3_lo, const int *q3_hi,
-                double cdtdx1, double cdtdx2, double cdtdx3);
-void orc_apply_av(const int *lo, const int *hi, int idir, const double *dx,
-                  const double *div, const int *d_lo, const int *d_hi,
-                  const double *u, const int *u_lo, const int *u_hi,
-                  double *flux, const int *f_lo, const int *f_hi);
+                const real *qm, const int *qm_lo, const int *qm_hi,
+                real *qmo, const int *qmo_lo, const int *qmo_hi,
+                const real *qp, const int *qp_lo, const int *qp_hi,
+                real *qpo, const int *qpo_lo, const int *qpo_hi,
+                const real *qaux, const int *qa_lo, const int *qa_hi,
+                const real *f2, const int *f2_lo, const int *f2_hi,
+                const real *f3, const int *f3_lo, const int *f3_hi,
+                const real *q2, const int *q2_lo, const int *q2_hi,
+                const real *q3, const int *q3_lo, const int *q3_hi,
+                real cdtdx1, real cdtdx2, real cdtdx3);
+void orc_apply_av(const int *lo, const int *hi, int idir, const real *dx,
+                  const real *div, const int *d_lo, const int *d_hi,
+                  const real *u, const int *u_lo, const int *u_hi,
+                  real *flux, const int *f_lo, const int *f_hi);
 void orc_normalize_species_fluxes(const int *lo, const int *hi,
-                                  double *flux, const int *f_lo, const int *f_hi);
+                                  real *flux, const int *f_lo, const int *f_hi);
 void orc_store_flux(const int *lo, const int *hi,
-                    double *fout, const int *fo_lo, const int *fo_hi,
-                    const double *fin, const int *fi_lo, const int *fi_hi,
-                    const double *area, const int *a_lo, const int *a_hi, double dt);
+                    real *fout, const int *fo_lo, const int *fo_hi,
+                    const real *fin, const int *fi_lo, const int *fi_hi,
+                    const real *area, const int *a_lo, const int *a_hi, real dt);
 void orc_fill_hydro_source(const int *lo, const int *hi,
-                           const double *u, const int *u_lo, const int *u_hi,
-                           const double *q, const int *q_lo, const int *q_hi,
-                           double *src, const int *s_lo, const int *s_hi,
-                           const double *f1, const int *f1_lo, const int *f1_hi,
-                           const double *f2, const int *f2_lo, const int *f2_hi,
-                           const double *f3, const int *f3_lo, const int *f3_hi,
-                           const double *qx, const int *qx_lo, const int *qx_hi,
-                           const double *qy, const int *qy_lo, const int *qy_hi,
-                           const double *qz, const int *qz_lo, const int *qz_hi,
-                           const double *a1, const int *a1_lo, const int *a1_hi,
-                           const double *a2, const int *a2_lo, const int *a2_hi,
-                           const double *a3, const int *a3_lo, const int *a3_hi,
-                           const double *vol, const int *v_lo, const int *v_hi,
-                           const double *dx, double dt);
-void orc_estdt(const int *lo, const int *hi, const double *u, const int *u_lo, const int *u_hi,
-               const double *dx, double *dt);
-void orc_enforce_minimum_density(const int *lo, const int *hi, double *u, const int *u_lo, const int *u_hi);
-void orc_normalize_species(const int *lo, const int *hi, double *u, const int *u_lo, const int *u_hi);
-void orc_reset_internal_e(const int *lo, const int *hi, double *u, const int *u_lo, const int *u_hi);
-void orc_compute_temp(const int *lo, const int *hi, double *u, const int *u_lo, const int *u_hi);
+                           const real *u, const int *u_lo, const int *u_hi,
+                           const real *q, const int *q_lo, const int *q_hi,
+                           real *src, const int *s_lo, const int *s_hi,
+                           const real *f1, const int *f1_lo, const int *f1_hi,
+                           const real *f2, const int *f2_lo, const int *f2_hi,
+                           const real *f3, const int *f3_lo, const int *f3_hi,
+                           const real *qx, const int *qx_lo, const int *qx_hi,
+                           const real *qy, const int *qy_lo, const int *qy_hi,
+                           const real *qz, const int *qz_lo, const int *qz_hi,
+                           const real *a1, const int *a1_lo, const int *a1_hi,
+                           const real *a2, const int *a2_lo, const int *a2_hi,
+                           const real *a3, const int *a3_lo, const int *a3_hi,
+                           const real *vol, const int *v_lo, const int *v_hi,
+                           const real *dx, real dt);
+void orc_estdt(const int *lo, const int *hi, const real *u, const int *u_lo, const int *u_hi,
+               const real *dx, real *dt);
+void orc_enforce_minimum_density(const int *lo, const int *hi, real *u, const int *u_lo, const int *u_hi);
+void orc_normalize_species(const int *lo, const int *hi, real *u, const int *u_lo, const int *u_hi);
+void orc_reset_internal_e(const int *lo, const int *hi, real *u, const int *u_lo, const int *u_hi);
+void orc_compute_temp(const int *lo, const int *hi, real *u, const int *u_lo, const int *u_hi);
 
 /* Problem set-up and diagnostics (Source/Prob.F90, Source/Castro_nd.F90:301-400) */
 typedef struct {
-    double p_ambient, dens_ambient, exp_energy, r_init;
+    real p_ambient, dens_ambient, exp_energy, r_init;
     int nsub, pad_;
 } orc_prob_t;
-void orc_initdata(const int *lo, const int *hi, double *u, const int *u_lo, const int *u_hi,
-                  const double *dx, const double *problo, const double *probhi,
+void orc_initdata(const int *lo, const int *hi, real *u, const int *u_lo, const int *u_hi,
+                  const real *dx, const real *problo, const real *probhi,
                   const orc_prob_t *prob);
 void orc_calculate_blast_radius(const int *lo, const int *hi,
-                                const double *u, const int *u_lo, const int *u_hi,
-                                const double *dx, const double *problo, const double *probhi,
-                                double *blast_mass, double *blast_radius, double max_density);
+                                const real *u, const int *u_lo, const int *u_hi,
+                                const real *dx, const real *problo, const real *probhi,
+                                real *blast_mass, real *blast_radius, real max_density);
 void orc_denerror(const int *lo, const int *hi, signed char *tag, const int *t_lo, const int *t_hi,
-                  const double *den, const int *d_lo, const int *d_hi,
+                  const real *den, const int *d_lo, const int *d_hi,
                   signed char set, signed char clear);
 
 /* Level driver (Source/Castro_advance.cpp, Castro_hydro.cpp, Castro.cpp) */
 typedef struct orc_level orc_level_t;
 enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1 };
 /* boxes: nbox x 6 ints (lo xyz, hi xyz) tiling [0, n_cell) */
-orc_level_t *orc_level_create(const int *n_cell, const double *prob_lo, const double *prob_hi,
+orc_level_t *orc_level_create(const int *n_cell, const real *prob_lo, const real *prob_hi,
                               int bc, int nbox, const int *boxes, int nthreads);
 void orc_level_destroy(orc_level_t *L);
 int orc_level_nbox(const orc_level_t *L);
-double *orc_level_state(orc_level_t *L, int ibox);    /* S_new of box ibox, 20 comps, no ghosts */
+real *orc_level_state(orc_level_t *L, int ibox);    /* S_new of box ibox, 20 comps, no ghosts */
 void orc_level_init_data(orc_level_t *L, const orc_prob_t *prob);
-double orc_level_est_time_step(orc_level_t *L);        /* Castro.cpp:175-216 */
-double orc_level_initial_dt(orc_level_t *L);           /* Castro.cpp:162-173 (init_shrink) */
-void orc_level_advance(orc_level_t *L, double dt);     /* Castro_advance.cpp:11-100 */
+real orc_level_est_time_step(orc_level_t *L);        /* Castro.cpp:175-216 */
+real orc_level_initial_dt(orc_level_t *L);           /* Castro.cpp:162-173 (init_shrink) */
+void orc_level_advance(orc_level_t *L, real dt);     /* Castro_advance.cpp:11-100 */
 void orc_level_post_timestep(orc_level_t *L);          /* Castro.cpp:343-369 */
 void orc_level_clean_state_new(orc_level_t *L);        /* Castro.cpp:546-593 on S_new */
 /* One coarse time step, Amr::coarseTimeStep semantics: returns dt taken. */
-double orc_level_coarse_step(orc_level_t *L, double stop_time);
+real orc_level_coarse_step(orc_level_t *L, real stop_time);
 int orc_level_steps(const orc_level_t *L);
-double orc_level_time(const orc_level_t *L);
-double orc_level_dt(const orc_level_t *L);
+real orc_level_time(const orc_level_t *L);
+real orc_level_dt(const orc_level_t *L);
 /* hydro source of the last advance, for debugging (box ibox) */
-double *orc_level_source(orc_level_t *L, int ibox);
+real *orc_level_source(orc_level_t *L, int ibox);
 
 #ifdef __cplusplus
 }

@@ -225,3 +225,90 @@ class Level:
     @property
     def dt(self):
         return self.L.orc_level_dt(self.h)
+
+
+# ------------------------------------------------------------ tangent-linear
+# build/liboracle_tl.so: the same restatement with real = (value, derivative)
+# dual numbers.  A state array holds NVAR x cells duals, i.e. pairs of doubles.
+TL_PATH = os.path.join(HERE, "build", "liboracle_tl.so")
+
+
+class DualT(C.Structure):
+    _fields_ = [("v", C.c_double), ("d", C.c_double)]
+
+
+_tl = None
+
+
+def tl_lib():
+    global _tl
+    if _tl is None:
+        if not os.path.exists(TL_PATH):
+            build()
+        L = C.CDLL(TL_PATH)
+        L.orc_level_create.restype = C.c_void_p
+        L.orc_level_create.argtypes = [_ip, _dp, _dp, C.c_int, C.c_int, _ip, C.c_int]
+        L.orc_level_state.restype = _dp
+        L.orc_level_state.argtypes = [C.c_void_p, C.c_int]
+        L.orc_level_advance.argtypes = [C.c_void_p, DualT]
+        L.orc_level_post_timestep.argtypes = [C.c_void_p]
+        L.orc_level_destroy.argtypes = [C.c_void_p]
+        L.orc_eos_init_arrays.argtypes = [_dp]
+        _tl = L
+    return _tl
+
+
+def tl_eos_init_arrays(tab):
+    t = np.ascontiguousarray(tab, dtype=np.float64)
+    tl_lib().orc_eos_init_arrays(t.ctypes.data_as(_dp))
+
+
+class TLLevel:
+    """Tangent-linear of the oracle level: states carry one directional
+    derivative.  advance(dt) + post_timestep() is one step with a given dt."""
+
+    def __init__(self, n_cell, prob_lo, prob_hi, bc, boxes, nthreads=1):
+        self.L = tl_lib()
+        boxes = np.asarray(boxes, dtype=np.int32).reshape(-1, 6)
+        self.boxes = boxes
+        bx = np.ascontiguousarray(boxes.ravel())
+        lo = np.ascontiguousarray(np.stack([np.asarray(prob_lo, float), np.zeros(3)], axis=1))
+        hi = np.ascontiguousarray(np.stack([np.asarray(prob_hi, float), np.zeros(3)], axis=1))
+        self.h = self.L.orc_level_create(iv(*n_cell), lo.ctypes.data_as(_dp), hi.ctypes.data_as(_dp), int(bc),
+                                         len(boxes), bx.ctypes.data_as(_ip), int(nthreads))
+        self.n_cell = tuple(n_cell)
+
+    def __del__(self):
+        try:
+            self.L.orc_level_destroy(self.h)
+        except Exception:
+            pass
+
+    def _box(self, b):
+        lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+        n = int(np.prod(hi - lo + 1)) * NVAR
+        return np.ctypeslib.as_array(self.L.orc_level_state(self.h, b), shape=(n, 2))
+
+    def set_state(self, S, dS):
+        for b in range(len(self.boxes)):
+            lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+            v = self._box(b)
+            sl = (slice(lo[0], hi[0] + 1), slice(lo[1], hi[1] + 1), slice(lo[2], hi[2] + 1))
+            v[:, 0] = np.asfortranarray(S[sl]).ravel(order="F")
+            v[:, 1] = np.asfortranarray(dS[sl]).ravel(order="F")
+
+    def state(self):
+        out = np.zeros(self.n_cell + (NVAR,), order="F")
+        dout = np.zeros(self.n_cell + (NVAR,), order="F")
+        for b in range(len(self.boxes)):
+            lo, hi = self.boxes[b][:3], self.boxes[b][3:]
+            shp = tuple(hi - lo + 1) + (NVAR,)
+            v = self._box(b)
+            sl = (slice(lo[0], hi[0] + 1), slice(lo[1], hi[1] + 1), slice(lo[2], hi[2] + 1))
+            out[sl] = v[:, 0].reshape(shp, order="F")
+            dout[sl] = v[:, 1].reshape(shp, order="F")
+        return out, dout
+
+    def step(self, dt):
+        self.L.orc_level_advance(self.h, DualT(float(dt), 0.0))
+        self.L.orc_level_post_timestep(self.h)

@@ -114,20 +114,20 @@ void orc_eos_finalize(void)
 }
 
 /* Quintic Hermite basis, Source/eos.F90:740-911 */
-static double psi0(double z) { return (z * z * z) * (z * (-6.0 * z + 15.0) - 10.0) + 1.0; }
-static double dpsi0(double z) { return (z * z) * (z * (-30.0 * z + 60.0) - 30.0); }
-static double ddpsi0(double z) { return z * (z * (-120.0 * z + 180.0) - 60.0); }
-static double psi1(double z) { return z * ((z * z) * (z * (-3.0 * z + 8.0) - 6.0) + 1.0); }
-static double dpsi1(double z) { return z * z * (z * (-15.0 * z + 32.0) - 18.0) + 1.0; }
-static double ddpsi1(double z) { return z * (z * (-60.0 * z + 96.0) - 36.0); }
-static double psi2(double z) { return 0.5 * z * z * (z * (z * (-z + 3.0) - 3.0) + 1.0); }
-static double dpsi2(double z) { return 0.5 * z * (z * (z * (-5.0 * z + 12.0) - 9.0) + 2.0); }
-static double ddpsi2(double z) { return 0.5 * (z * (z * (-20.0 * z + 36.0) - 18.0) + 2.0); }
+static real psi0(real z) { return (z * z * z) * (z * (-6.0 * z + 15.0) - 10.0) + 1.0; }
+static real dpsi0(real z) { return (z * z) * (z * (-30.0 * z + 60.0) - 30.0); }
+static real ddpsi0(real z) { return z * (z * (-120.0 * z + 180.0) - 60.0); }
+static real psi1(real z) { return z * ((z * z) * (z * (-3.0 * z + 8.0) - 6.0) + 1.0); }
+static real dpsi1(real z) { return z * z * (z * (-15.0 * z + 32.0) - 18.0) + 1.0; }
+static real ddpsi1(real z) { return z * (z * (-60.0 * z + 96.0) - 36.0); }
+static real psi2(real z) { return 0.5 * z * z * (z * (z * (-z + 3.0) - 3.0) + 1.0); }
+static real dpsi2(real z) { return 0.5 * z * (z * (z * (-5.0 * z + 12.0) - 9.0) + 2.0); }
+static real ddpsi2(real z) { return 0.5 * (z * (z * (-20.0 * z + 36.0) - 18.0) + 2.0); }
 
 /* Biquintic Hermite, Source/eos.F90:915-950 */
-static double h5(const double *fi, double w0t, double w1t, double w2t, double w0mt, double w1mt,
-                 double w2mt, double w0d, double w1d, double w2d, double w0md, double w1md,
-                 double w2md)
+static real h5(const double *fi, real w0t, real w1t, real w2t, real w0mt, real w1mt,
+                 real w2mt, real w0d, real w1d, real w2d, real w0md, real w1md,
+                 real w2md)
 {
     return fi[0] * w0d * w0t + fi[1] * w0md * w0t + fi[2] * w0d * w0mt + fi[3] * w0md * w0mt +
            fi[4] * w0d * w1t + fi[5] * w0md * w1t + fi[6] * w0d * w1mt + fi[7] * w0md * w1mt +
@@ -141,14 +141,14 @@ static double h5(const double *fi, double w0t, double w1t, double w2t, double w0
 }
 
 /* Cubic Hermite basis, Source/eos.F90:955-1031 */
-static double xpsi0(double z) { return z * z * (2.0 * z - 3.0) + 1.0; }
-static double xdpsi0(double z) { return z * (6.0 * z - 6.0); }
-static double xpsi1(double z) { return z * (z * (z - 2.0) + 1.0); }
-static double xdpsi1(double z) { return z * (3.0 * z - 4.0) + 1.0; }
+static real xpsi0(real z) { return z * z * (2.0 * z - 3.0) + 1.0; }
+static real xdpsi0(real z) { return z * (6.0 * z - 6.0); }
+static real xpsi1(real z) { return z * (z * (z - 2.0) + 1.0); }
+static real xdpsi1(real z) { return z * (3.0 * z - 4.0) + 1.0; }
 
 /* Bicubic Hermite, Source/eos.F90:1034-1059 */
-static double h3(const double *fi, double w0t, double w1t, double w0mt, double w1mt, double w0d,
-                 double w1d, double w0md, double w1md)
+static real h3(const double *fi, real w0t, real w1t, real w0mt, real w1mt, real w0d,
+                 real w1d, real w0md, real w1md)
 {
     return fi[0] * w0d * w0t + fi[1] * w0md * w0t + fi[2] * w0d * w0mt + fi[3] * w0md * w0mt +
            fi[4] * w0d * w1t + fi[5] * w0md * w1t + fi[6] * w0d * w1mt + fi[7] * w0md * w1mt +
@@ -168,21 +168,21 @@ static void gather4(double *fi, int c, int iat, int jat)
 void orc_eos(int input, orc_eos_t *state)
 {
     int converged, iter, iat = 0, jat = 0, niter = 0;
-    double temp_old, v_want = 0.0, v, dvdx, error;
-    double temp, den, din, deni, tempi, ytot1, ye;
-    double pres, ener, entr, dpresdd, dpresdt, denerdd, denerdt, dentrdd, dentrdt;
-    double pele, dpepdt, dpepdd, eele, deepdt, deepdd, sele, dsepdd, dsepdt;
-    double prad, dpraddd, dpraddt, erad, deraddd, deraddt, srad, dsraddd, dsraddt;
-    double pion, dpiondd, dpiondt, eion, deiondd, deiondt, sion, dsiondd, dsiondt;
-    double s, x, y, z, zz, zzi, chit, chid;
-    double free, df_d, df_t, df_tt, df_dt;
-    double xt, xd, mxt, mxd;
+    real temp_old, v_want = 0.0, v, dvdx, error;
+    real temp, den, din, deni, tempi, ytot1, ye;
+    real pres, ener, entr, dpresdd, dpresdt, denerdd, denerdt, dentrdd, dentrdt;
+    real pele, dpepdt, dpepdd, eele, deepdt, deepdd, sele, dsepdd, dsepdt;
+    real prad, dpraddd, dpraddt, erad, deraddd, deraddt, srad, dsraddd, dsraddt;
+    real pion, dpiondd, dpiondt, eion, deiondd, deiondt, sion, dsiondd, dsiondt;
+    real s, x, y, z, zz, zzi, chit, chid;
+    real free, df_d, df_t, df_tt, df_dt;
+    real xt, xd, mxt, mxd;
     double fi[36];
-    double si0t, si1t, si2t, si0mt, si1mt, si2mt;
-    double si0d, si1d, si2d, si0md, si1md, si2md;
-    double dsi0t, dsi1t, dsi2t, dsi0mt, dsi1mt, dsi2mt;
-    double dsi0d, dsi1d, dsi2d, dsi0md, dsi1md, dsi2md;
-    double ddsi0t, ddsi1t, ddsi2t, ddsi0mt, ddsi1mt, ddsi2mt;
+    real si0t, si1t, si2t, si0mt, si1mt, si2mt;
+    real si0d, si1d, si2d, si0md, si1md, si2md;
+    real dsi0t, dsi1t, dsi2t, dsi0mt, dsi1mt, dsi2mt;
+    real dsi0d, dsi1d, dsi2d, dsi0md, dsi1md, dsi2md;
+    real ddsi0t, ddsi1t, ddsi2t, ddsi0mt, ddsi1mt, ddsi2mt;
 
     converged = (input == EOS_INPUT_RT); /* :181-185 */
     if (input == EOS_INPUT_RE) v_want = state->e;

@@ -35,22 +35,22 @@ static long npts(box_t b)
 {
     return (long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
 }
-static double *alloc_fab(box_t b, int nc) { return (double *)calloc((size_t)npts(b) * nc, sizeof(double)); }
+static real *alloc_fab(box_t b, int nc) { return (real *)calloc((size_t)npts(b) * nc, sizeof(real)); }
 
 struct orc_level {
     int n_cell[3], bc, nbox, nthreads;
-    double prob_lo[3], prob_hi[3], dx[3];
+    real prob_lo[3], prob_hi[3], dx[3];
     box_t *box;
-    double **S_new, **S_old, **Sborder, **src;
+    real **S_new, **S_old, **Sborder, **src;
     int *owner;          /* owner box of each domain cell */
     int steps;
-    double time, dt;
+    real time, dt;
     int tile[3];
 };
 
 static box_t vbox(const orc_level_t *L, int b) { return L->box[b]; }
 
-orc_level_t *orc_level_create(const int *n_cell, const double *prob_lo, const double *prob_hi,
+orc_level_t *orc_level_create(const int *n_cell, const real *prob_lo, const real *prob_hi,
                               int bc, int nbox, const int *boxes, int nthreads)
 {
     orc_level_t *L = (orc_level_t *)calloc(1, sizeof(orc_level_t));
@@ -58,17 +58,17 @@ orc_level_t *orc_level_create(const int *n_cell, const double *prob_lo, const do
         L->n_cell[d] = n_cell[d];
         L->prob_lo[d] = prob_lo[d];
         L->prob_hi[d] = prob_hi[d];
-        L->dx[d] = (prob_hi[d] - prob_lo[d]) / (double)n_cell[d]; /* Geometry::CellSize */
+        L->dx[d] = (prob_hi[d] - prob_lo[d]) / (real)n_cell[d]; /* Geometry::CellSize */
     }
     L->bc = bc;
     L->nbox = nbox;
     L->nthreads = nthreads > 0 ? nthreads : 1;
     L->tile[0] = 1024; L->tile[1] = 16; L->tile[2] = 16;
     L->box = (box_t *)malloc(sizeof(box_t) * nbox);
-    L->S_new = (double **)calloc(nbox, sizeof(double *));
-    L->S_old = (double **)calloc(nbox, sizeof(double *));
-    L->Sborder = (double **)calloc(nbox, sizeof(double *));
-    L->src = (double **)calloc(nbox, sizeof(double *));
+    L->S_new = (real **)calloc(nbox, sizeof(real *));
+    L->S_old = (real **)calloc(nbox, sizeof(real *));
+    L->Sborder = (real **)calloc(nbox, sizeof(real *));
+    L->src = (real **)calloc(nbox, sizeof(real *));
     long ncell = (long)n_cell[0] * n_cell[1] * n_cell[2];
     L->owner = (int *)malloc(sizeof(int) * ncell);
     for (long c = 0; c < ncell; ++c) L->owner[c] = -1;
@@ -97,11 +97,11 @@ void orc_level_destroy(orc_level_t *L)
 }
 
 int orc_level_nbox(const orc_level_t *L) { return L->nbox; }
-double *orc_level_state(orc_level_t *L, int b) { return L->S_new[b]; }
-double *orc_level_source(orc_level_t *L, int b) { return L->src[b]; }
+real *orc_level_state(orc_level_t *L, int b) { return L->S_new[b]; }
+real *orc_level_source(orc_level_t *L, int b) { return L->src[b]; }
 int orc_level_steps(const orc_level_t *L) { return L->steps; }
-double orc_level_time(const orc_level_t *L) { return L->time; }
-double orc_level_dt(const orc_level_t *L) { return L->dt; }
+real orc_level_time(const orc_level_t *L) { return L->time; }
+real orc_level_dt(const orc_level_t *L) { return L->dt; }
 
 /* MFIter tiles of a box: tile t of nt */
 static int ntiles(const orc_level_t *L, box_t b, int *nt3)
@@ -145,7 +145,7 @@ static int make_work(const orc_level_t *L, work_t **out)
 /* Castro::clean_state, Source/Castro.cpp:546-593: the four kernels over
    growntilebox(ng).  All four are pointwise, so the growth of the edge tiles
    is restated by cleaning the whole grown box in slabs. */
-static void clean_state(orc_level_t *L, double **S, int ng)
+static void clean_state(orc_level_t *L, real **S, int ng)
 {
 #pragma omp parallel for schedule(dynamic) num_threads(L->nthreads)
     for (int b = 0; b < L->nbox; ++b) {
@@ -187,35 +187,35 @@ static void fill_patch(orc_level_t *L)
     }
 }
 
-static double *const_fab(box_t b, double v)
+static real *const_fab(box_t b, real v)
 {
     long n = npts(b);
-    double *p = (double *)malloc(sizeof(double) * n);
+    real *p = (real *)malloc(sizeof(real) * n);
     for (long c = 0; c < n; ++c) p[c] = v;
     return p;
 }
 
 /* Source/Castro_hydro.cpp:31-397 for one tile (bx) of box ib */
-static void hydro_tile(orc_level_t *L, int ib, box_t bx, double dt)
+static void hydro_tile(orc_level_t *L, int ib, box_t bx, real dt)
 {
-    const double *dx = L->dx;
+    const real *dx = L->dx;
     const int *domlo = (const int[3]){0, 0, 0};
     const int domhi[3] = {L->n_cell[0] - 1, L->n_cell[1] - 1, L->n_cell[2] - 1};
     box_t fb = grow1(L->box[ib], 4); /* Sborder FAB */
-    double *state = L->Sborder[ib];
-    double *source = L->src[ib];
+    real *state = L->Sborder[ib];
+    real *source = L->src[ib];
     box_t sb = L->box[ib];
     box_t obx = grow1(bx, 1), qbx = grow1(bx, 4);
     box_t ebx[3], gebx[3], tbx[3][3];
 
-    double *q = alloc_fab(qbx, ORC_QVAR), *qaux = alloc_fab(qbx, ORC_NQAUX);
+    real *q = alloc_fab(qbx, ORC_QVAR), *qaux = alloc_fab(qbx, ORC_NQAUX);
     orc_ctoprim(qbx.lo, qbx.hi, state, fb.lo, fb.hi, q, qbx.lo, qbx.hi, qaux, qbx.lo, qbx.hi);
 
-    double *flux[3], *qe[3], *area[3], *fluxes_out[3];
+    real *flux[3], *qe[3], *area[3], *fluxes_out[3];
     box_t areab[3], volb = grow1(L->box[ib], 4);
     /* Geometry::GetFaceArea / GetVolume for Cartesian coordinates */
-    const double ar[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
-    double *vol = const_fab(volb, dx[0] * dx[1] * dx[2]);
+    const real ar[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
+    real *vol = const_fab(volb, dx[0] * dx[1] * dx[2]);
     for (int d = 0; d < 3; ++d) {
         ebx[d] = nodes(bx, d);
         gebx[d] = grow1(ebx[d], 1);
@@ -235,22 +235,22 @@ static void hydro_tile(orc_level_t *L, int ib, box_t bx, double dt)
     tbx[2][1] = grow(ebx[2], 1, 0, 0);
     tbx[2][2] = grow(ebx[2], 1, 1, 0);
 
-    double *qm[3][3], *qp[3][3];
+    real *qm[3][3], *qp[3][3];
     for (int a = 0; a < 3; ++a)
         for (int c = 0; c < 3; ++c) {
             qm[a][c] = alloc_fab(tbx[a][c], ORC_QVAR);
             qp[a][c] = alloc_fab(tbx[a][c], ORC_QVAR);
         }
-    double *div = alloc_fab(obx, 1);
+    real *div = alloc_fab(obx, 1);
     orc_divu(obx.lo, obx.hi, q, qbx.lo, qbx.hi, dx, div, obx.lo, obx.hi);
 
-    double *q_int = alloc_fab(obx, ORC_QVAR);
-    double *ftmp1 = alloc_fab(obx, ORC_NVAR), *ftmp2 = alloc_fab(obx, ORC_NVAR);
-    double *qg1 = alloc_fab(obx, ORC_NGDNV), *qg2 = alloc_fab(obx, ORC_NGDNV);
-    double *ql = alloc_fab(obx, ORC_QVAR), *qr = alloc_fab(obx, ORC_QVAR);
+    real *q_int = alloc_fab(obx, ORC_QVAR);
+    real *ftmp1 = alloc_fab(obx, ORC_NVAR), *ftmp2 = alloc_fab(obx, ORC_NVAR);
+    real *qg1 = alloc_fab(obx, ORC_NGDNV), *qg2 = alloc_fab(obx, ORC_NGDNV);
+    real *ql = alloc_fab(obx, ORC_QVAR), *qr = alloc_fab(obx, ORC_QVAR);
 
-    const double hdtdx[3] = {0.5 * dt / dx[0], 0.5 * dt / dx[1], 0.5 * dt / dx[2]};
-    const double cdtdx[3] = {dt / dx[0] / 3.0, dt / dx[1] / 3.0, dt / dx[2] / 3.0};
+    const real hdtdx[3] = {0.5 * dt / dx[0], 0.5 * dt / dx[1], 0.5 * dt / dx[2]};
+    const real cdtdx[3] = {dt / dx[0] / 3.0, dt / dx[1] / 3.0, dt / dx[2] / 3.0};
 
     for (int d = 0; d < 3; ++d)
         orc_trace_ppm(obx.lo, obx.hi, bx.lo, bx.hi, d + 1, q, qbx.lo, qbx.hi, qaux, qbx.lo, qbx.hi,
@@ -309,9 +309,9 @@ static void hydro_tile(orc_level_t *L, int ib, box_t bx, double dt)
 }
 
 /* Castro::construct_hydro_source, Source/Castro_hydro.cpp:6-399 */
-static void construct_hydro_source(orc_level_t *L, double dt)
+static void construct_hydro_source(orc_level_t *L, real dt)
 {
-    for (int b = 0; b < L->nbox; ++b) memset(L->src[b], 0, sizeof(double) * npts(L->box[b]) * ORC_NVAR);
+    for (int b = 0; b < L->nbox; ++b) memset(L->src[b], 0, sizeof(real) * npts(L->box[b]) * ORC_NVAR);
     work_t *w;
     int nw = make_work(L, &w);
 #pragma omp parallel for schedule(dynamic) num_threads(L->nthreads)
@@ -320,20 +320,20 @@ static void construct_hydro_source(orc_level_t *L, double dt)
 }
 
 /* Castro::advance, Source/Castro_advance.cpp:11-100 (single level) */
-void orc_level_advance(orc_level_t *L, double dt)
+void orc_level_advance(orc_level_t *L, real dt)
 {
-    double **tmp = L->S_old; /* swapTimeLevels, :32-33 */
+    real **tmp = L->S_old; /* swapTimeLevels, :32-33 */
     L->S_old = L->S_new;
     L->S_new = tmp;
     clean_state(L, L->S_old, 0); /* :51 */
     for (int b = 0; b < L->nbox; ++b) /* MultiFab::Copy, :55 */
-        memcpy(L->S_new[b], L->S_old[b], sizeof(double) * npts(L->box[b]) * ORC_NVAR);
+        memcpy(L->S_new[b], L->S_old[b], sizeof(real) * npts(L->box[b]) * ORC_NVAR);
     fill_patch(L);                /* :62 */
     clean_state(L, L->Sborder, 4); /* :66 */
     construct_hydro_source(L, dt); /* :70 */
     for (int b = 0; b < L->nbox; ++b) { /* MultiFab::Saxpy, :74 */
         long n = npts(L->box[b]) * ORC_NVAR;
-        double *s = L->S_new[b], *h = L->src[b];
+        real *s = L->S_new[b], *h = L->src[b];
         for (long c = 0; c < n; ++c) s[c] = s[c] + dt * h[c];
     }
     clean_state(L, L->S_new, 0); /* :78 */
@@ -342,10 +342,10 @@ void orc_level_advance(orc_level_t *L, double dt)
 void orc_level_post_timestep(orc_level_t *L) { clean_state(L, L->S_new, 0); } /* Castro.cpp:369 */
 
 /* Castro::estTimeStep, Source/Castro.cpp:175-216 */
-double orc_level_est_time_step(orc_level_t *L)
+real orc_level_est_time_step(orc_level_t *L)
 {
-    double dt = DBL_MAX;
-    double *dtb = (double *)malloc(sizeof(double) * L->nbox);
+    real dt = DBL_MAX;
+    real *dtb = (real *)malloc(sizeof(real) * L->nbox);
 #pragma omp parallel for schedule(dynamic) num_threads(L->nthreads)
     for (int b = 0; b < L->nbox; ++b) {
         box_t bb = vbox(L, b);
@@ -354,15 +354,15 @@ double orc_level_est_time_step(orc_level_t *L)
     }
     for (int b = 0; b < L->nbox; ++b) dt = fmin(dt, dtb[b]);
     free(dtb);
-    const double cfl = 0.5;
+    const real cfl = 0.5;
     dt *= cfl;
     return dt;
 }
 
 /* Castro::initialTimeStep, Source/Castro.cpp:162-173 */
-double orc_level_initial_dt(orc_level_t *L)
+real orc_level_initial_dt(orc_level_t *L)
 {
-    const double init_shrink = 0.01;
+    const real init_shrink = 0.01;
     return init_shrink * orc_level_est_time_step(L);
 }
 
@@ -372,7 +372,7 @@ void orc_level_init_data(orc_level_t *L, const orc_prob_t *prob)
 #pragma omp parallel for schedule(dynamic) num_threads(L->nthreads)
     for (int b = 0; b < L->nbox; ++b) {
         box_t bb = vbox(L, b);
-        memset(L->S_new[b], 0, sizeof(double) * npts(bb) * ORC_NVAR);
+        memset(L->S_new[b], 0, sizeof(real) * npts(bb) * ORC_NVAR);
         orc_initdata(bb.lo, bb.hi, L->S_new[b], bb.lo, bb.hi, L->dx, L->prob_lo, L->prob_hi, prob);
     }
     L->steps = 0;
@@ -383,18 +383,18 @@ void orc_level_init_data(orc_level_t *L, const orc_prob_t *prob)
 /* Amr::coarseTimeStep with Castro::computeInitialDt / computeNewDt
    (Source/Castro.cpp:218-341) on a single level, then Amr::timeStep:
    advance + post_timestep. */
-double orc_level_coarse_step(orc_level_t *L, double stop_time)
+real orc_level_coarse_step(orc_level_t *L, real stop_time)
 {
-    double dt_0;
+    real dt_0;
     if (L->steps > 0) {
-        const double change_max = 1.1e0;
-        double dt_min = orc_level_est_time_step(L);
+        const real change_max = 1.1e0;
+        real dt_min = orc_level_est_time_step(L);
         dt_min = fmin(dt_min, change_max * L->dt);
         dt_0 = fmin(1.0e+100, dt_min);
     } else {
         dt_0 = fmin(1.0e+100, orc_level_initial_dt(L));
     }
-    const double eps = 0.001 * dt_0;
+    const real eps = 0.001 * dt_0;
     if (stop_time >= 0.0)
         if ((L->time + dt_0) > (stop_time - eps)) dt_0 = stop_time - L->time;
     L->dt = dt_0;

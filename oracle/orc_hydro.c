@@ -11,15 +11,15 @@
 
 /* Fortran-ordered array view: a(i,j,k,n), n 0-based. */
 typedef struct {
-    double *p;
+    real *p;
     int lo0, lo1, lo2;
     long s1, s2, s3;
 } arr_t;
 
-static arr_t mk(const double *p, const int *lo, const int *hi)
+static arr_t mk(const real *p, const int *lo, const int *hi)
 {
     arr_t a;
-    a.p = (double *)p;
+    a.p = (real *)p;
     a.lo0 = lo[0];
     a.lo1 = lo[1];
     a.lo2 = lo[2];
@@ -32,30 +32,30 @@ static arr_t mk(const double *p, const int *lo, const int *hi)
     ((a).p[((i) - (a).lo0) + (a).s1 * ((j) - (a).lo1) + (a).s2 * ((k) - (a).lo2) + (a).s3 * (n)])
 
 /* Source/network.F90:7-21 */
-static const double aion[ORC_NSPEC] = {4.0, 12.0, 16.0, 20.0, 24.0, 28.0, 32.0,
+static const real aion[ORC_NSPEC] = {4.0, 12.0, 16.0, 20.0, 24.0, 28.0, 32.0,
                                        36.0, 40.0, 44.0, 48.0, 52.0, 56.0};
-static const double zion[ORC_NSPEC] = {2.0, 6.0, 8.0, 10.0, 12.0, 14.0, 16.0,
+static const real zion[ORC_NSPEC] = {2.0, 6.0, 8.0, 10.0, 12.0, 14.0, 16.0,
                                        18.0, 20.0, 22.0, 24.0, 26.0, 28.0};
-static double aion_inv(int n) { return 1.0 / aion[n]; }
+static real aion_inv(int n) { return 1.0 / aion[n]; }
 
 /* Source/Castro_nd.F90:59-65 */
-static const double small_dens = 1.0e-12;
-static const double small_temp = 1.0e3;
-static const double small_pres = 1.e-200;
-static const double smallu = 1.e-12;
-static const double small = 1.e-8;
-static const double dual_energy_eta1 = 1.e0;
+static const real small_dens = 1.0e-12;
+static const real small_temp = 1.0e3;
+static const real small_pres = 1.e-200;
+static const real smallu = 1.e-12;
+static const real small = 1.e-8;
+static const real dual_energy_eta1 = 1.e0;
 
 /* amrex_constants_module */
-static const double ZERO = 0.0, HALF = 0.5, ONE = 1.0, TWO = 2.0, THREE = 3.0, SIX = 6.0;
-static const double FOURTH = 0.25;
+static const real ZERO = 0.0, HALF = 0.5, ONE = 1.0, TWO = 2.0, THREE = 3.0, SIX = 6.0;
+static const real FOURTH = 0.25;
 #define SIXTH (1.0 / 6.0)
 #define TWO3RD (2.0 / 3.0)
 #define FOUR3RD (4.0 / 3.0)
-static const double M_PI_ = 3.141592653589793238462643383279502884197;
+static const real M_PI_ = 3.141592653589793238462643383279502884197;
 
 /* Fortran SIGN(ONE, x): |1| with the sign bit of x (signed zero honoured) */
-static double fsign1(double x) { return copysign(1.0, x); }
+static real fsign1(real x) { return copysign(1.0, x); }
 
 #define LOOP3(lo, hi) \
     for (int k = (lo)[2]; k <= (hi)[2]; ++k) \
@@ -64,24 +64,24 @@ static double fsign1(double x) { return copysign(1.0, x); }
 
 /* ------------------------------------------------------------------ */
 /* Source/Castro_hydro_nd.F90:11-69 */
-void orc_divu(const int *lo, const int *hi, const double *qp_, const int *q_lo, const int *q_hi,
-              const double *dx, double *divp, const int *d_lo, const int *d_hi)
+void orc_divu(const int *lo, const int *hi, const real *qp_, const int *q_lo, const int *q_hi,
+              const real *dx, real *divp, const int *d_lo, const int *d_hi)
 {
     arr_t q = mk(qp_, q_lo, q_hi), div = mk(divp, d_lo, d_hi);
-    double dxinv = ONE / dx[0], dyinv = ONE / dx[1], dzinv = ONE / dx[2];
+    real dxinv = ONE / dx[0], dyinv = ONE / dx[1], dzinv = ONE / dx[2];
     LOOP3(lo, hi)
     {
-        double ux = FOURTH *
+        real ux = FOURTH *
                     (+A(q, i, j, k, QU) - A(q, i - 1, j, k, QU) + A(q, i, j, k - 1, QU) -
                      A(q, i - 1, j, k - 1, QU) + A(q, i, j - 1, k, QU) - A(q, i - 1, j - 1, k, QU) +
                      A(q, i, j - 1, k - 1, QU) - A(q, i - 1, j - 1, k - 1, QU)) *
                     dxinv;
-        double vy = FOURTH *
+        real vy = FOURTH *
                     (+A(q, i, j, k, QV) - A(q, i, j - 1, k, QV) + A(q, i, j, k - 1, QV) -
                      A(q, i, j - 1, k - 1, QV) + A(q, i - 1, j, k, QV) - A(q, i - 1, j - 1, k, QV) +
                      A(q, i - 1, j, k - 1, QV) - A(q, i - 1, j - 1, k - 1, QV)) *
                     dyinv;
-        double wz = FOURTH *
+        real wz = FOURTH *
                     (+A(q, i, j, k, QW) - A(q, i, j, k - 1, QW) + A(q, i, j - 1, k, QW) -
                      A(q, i, j - 1, k - 1, QW) + A(q, i - 1, j, k, QW) - A(q, i - 1, j, k - 1, QW) +
                      A(q, i - 1, j - 1, k, QW) - A(q, i - 1, j - 1, k - 1, QW)) *
@@ -91,22 +91,22 @@ void orc_divu(const int *lo, const int *hi, const double *qp_, const int *q_lo, 
 }
 
 /* Source/Castro_hydro_nd.F90:73-170 */
-void orc_ctoprim(const int *lo, const int *hi, const double *up, const int *u_lo, const int *u_hi,
-                 double *qp_, const int *q_lo, const int *q_hi, double *qap, const int *qa_lo,
+void orc_ctoprim(const int *lo, const int *hi, const real *up, const int *u_lo, const int *u_hi,
+                 real *qp_, const int *q_lo, const int *q_hi, real *qap, const int *qa_lo,
                  const int *qa_hi)
 {
     arr_t u = mk(up, u_lo, u_hi), q = mk(qp_, q_lo, q_hi), qaux = mk(qap, qa_lo, qa_hi);
     LOOP3(lo, hi)
     {
         A(q, i, j, k, QRHO) = A(u, i, j, k, URHO);
-        double rhoinv = ONE / A(q, i, j, k, QRHO);
-        double vel0 = A(u, i, j, k, UMX) * rhoinv;
-        double vel1 = A(u, i, j, k, UMY) * rhoinv;
-        double vel2 = A(u, i, j, k, UMZ) * rhoinv;
+        real rhoinv = ONE / A(q, i, j, k, QRHO);
+        real vel0 = A(u, i, j, k, UMX) * rhoinv;
+        real vel1 = A(u, i, j, k, UMY) * rhoinv;
+        real vel2 = A(u, i, j, k, UMZ) * rhoinv;
         A(q, i, j, k, QU) = vel0;
         A(q, i, j, k, QV) = vel1;
         A(q, i, j, k, QW) = vel2;
-        double kineng = HALF * A(q, i, j, k, QRHO) *
+        real kineng = HALF * A(q, i, j, k, QRHO) *
                         (A(q, i, j, k, QU) * A(q, i, j, k, QU) + A(q, i, j, k, QV) * A(q, i, j, k, QV) +
                          A(q, i, j, k, QW) * A(q, i, j, k, QW));
         if ((A(u, i, j, k, UEDEN) - kineng) / A(u, i, j, k, UEDEN) > dual_energy_eta1)
@@ -120,7 +120,7 @@ void orc_ctoprim(const int *lo, const int *hi, const double *up, const int *u_lo
         st.T = A(q, i, j, k, QTEMP);
         st.rho = A(q, i, j, k, QRHO);
         st.e = A(q, i, j, k, QREINT);
-        double s1 = 0.0, s2 = 0.0;
+        real s1 = 0.0, s2 = 0.0;
         for (int n = 0; n < ORC_NSPEC; ++n) s1 = s1 + A(q, i, j, k, QFS + n) * aion_inv(n);
         st.abar = ONE / s1;
         for (int n = 0; n < ORC_NSPEC; ++n) s2 = s2 + A(q, i, j, k, QFS + n) * zion[n] * aion_inv(n);
@@ -140,17 +140,17 @@ void orc_ctoprim(const int *lo, const int *hi, const double *up, const int *u_lo
 
 /* Source/Castro_hydro_nd.F90:174-245 (keeps the duplicated div(i,j+1,k) /
    div(i+1,j,k) terms of :216-217 and :223-224) */
-void orc_apply_av(const int *lo, const int *hi, int idir, const double *dx, const double *divp,
-                  const int *d_lo, const int *d_hi, const double *up, const int *u_lo,
-                  const int *u_hi, double *fp, const int *f_lo, const int *f_hi)
+void orc_apply_av(const int *lo, const int *hi, int idir, const real *dx, const real *divp,
+                  const int *d_lo, const int *d_hi, const real *up, const int *u_lo,
+                  const int *u_hi, real *fp, const int *f_lo, const int *f_hi)
 {
     arr_t div = mk(divp, d_lo, d_hi), u = mk(up, u_lo, u_hi), flux = mk(fp, f_lo, f_hi);
-    const double difmag = 0.1;
+    const real difmag = 0.1;
     for (int n = 0; n < ORC_NVAR; ++n) {
         if (n == UTEMP) continue;
         LOOP3(lo, hi)
         {
-            double div1;
+            real div1;
             if (idir == 1) {
                 div1 = FOURTH * (A(div, i, j, k, 0) + A(div, i, j + 1, k, 0) + A(div, i, j, k + 1, 0) +
                                  A(div, i, j + 1, k, 0));
@@ -173,13 +173,13 @@ void orc_apply_av(const int *lo, const int *hi, int idir, const double *dx, cons
 }
 
 /* Source/Castro_hydro_nd.F90:248-297 */
-void orc_normalize_species_fluxes(const int *lo, const int *hi, double *fp, const int *f_lo,
+void orc_normalize_species_fluxes(const int *lo, const int *hi, real *fp, const int *f_lo,
                                   const int *f_hi)
 {
     arr_t flux = mk(fp, f_lo, f_hi);
     LOOP3(lo, hi)
     {
-        double sum = ZERO, fac;
+        real sum = ZERO, fac;
         for (int n = UFS; n < UFS + ORC_NSPEC; ++n) sum = sum + A(flux, i, j, k, n);
         if (sum != ZERO) fac = A(flux, i, j, k, URHO) / sum;
         else fac = ONE;
@@ -188,9 +188,9 @@ void orc_normalize_species_fluxes(const int *lo, const int *hi, double *fp, cons
 }
 
 /* Source/Castro_hydro_nd.F90:301-340 */
-void orc_store_flux(const int *lo, const int *hi, double *fop, const int *fo_lo, const int *fo_hi,
-                    const double *fip, const int *fi_lo, const int *fi_hi, const double *ap,
-                    const int *a_lo, const int *a_hi, double dt)
+void orc_store_flux(const int *lo, const int *hi, real *fop, const int *fo_lo, const int *fo_hi,
+                    const real *fip, const int *fi_lo, const int *fi_hi, const real *ap,
+                    const int *a_lo, const int *a_hi, real dt)
 {
     arr_t fo = mk(fop, fo_lo, fo_hi), fi = mk(fip, fi_lo, fi_hi), area = mk(ap, a_lo, a_hi);
     for (int n = 0; n < ORC_NVAR; ++n)
@@ -198,17 +198,17 @@ void orc_store_flux(const int *lo, const int *hi, double *fop, const int *fo_lo,
 }
 
 /* Source/Castro_hydro_nd.F90:344-441 */
-void orc_fill_hydro_source(const int *lo, const int *hi, const double *up, const int *u_lo,
-                           const int *u_hi, const double *qp_, const int *q_lo, const int *q_hi,
-                           double *sp, const int *s_lo, const int *s_hi, const double *f1p,
-                           const int *f1_lo, const int *f1_hi, const double *f2p, const int *f2_lo,
-                           const int *f2_hi, const double *f3p, const int *f3_lo, const int *f3_hi,
-                           const double *qxp, const int *qx_lo, const int *qx_hi, const double *qyp,
-                           const int *qy_lo, const int *qy_hi, const double *qzp, const int *qz_lo,
-                           const int *qz_hi, const double *a1p, const int *a1_lo, const int *a1_hi,
-                           const double *a2p, const int *a2_lo, const int *a2_hi, const double *a3p,
-                           const int *a3_lo, const int *a3_hi, const double *vp, const int *v_lo,
-                           const int *v_hi, const double *dx, double dt)
+void orc_fill_hydro_source(const int *lo, const int *hi, const real *up, const int *u_lo,
+                           const int *u_hi, const real *qp_, const int *q_lo, const int *q_hi,
+                           real *sp, const int *s_lo, const int *s_hi, const real *f1p,
+                           const int *f1_lo, const int *f1_hi, const real *f2p, const int *f2_lo,
+                           const int *f2_hi, const real *f3p, const int *f3_lo, const int *f3_hi,
+                           const real *qxp, const int *qx_lo, const int *qx_hi, const real *qyp,
+                           const int *qy_lo, const int *qy_hi, const real *qzp, const int *qz_lo,
+                           const int *qz_hi, const real *a1p, const int *a1_lo, const int *a1_hi,
+                           const real *a2p, const int *a2_lo, const int *a2_hi, const real *a3p,
+                           const int *a3_lo, const int *a3_hi, const real *vp, const int *v_lo,
+                           const int *v_hi, const real *dx, real dt)
 {
     (void)up; (void)u_lo; (void)u_hi; (void)qp_; (void)q_lo; (void)q_hi; (void)dt;
     arr_t src = mk(sp, s_lo, s_hi), f1 = mk(f1p, f1_lo, f1_hi), f2 = mk(f2p, f2_lo, f2_hi),
@@ -218,7 +218,7 @@ void orc_fill_hydro_source(const int *lo, const int *hi, const double *up, const
     for (int n = 0; n < ORC_NVAR; ++n)
         LOOP3(lo, hi)
         {
-            double volinv = ONE / A(vol, i, j, k, 0);
+            real volinv = ONE / A(vol, i, j, k, 0);
             A(src, i, j, k, n) =
                 A(src, i, j, k, n) +
                 (A(f1, i, j, k, n) * A(a1, i, j, k, 0) - A(f1, i + 1, j, k, n) * A(a1, i + 1, j, k, 0) +
@@ -226,7 +226,7 @@ void orc_fill_hydro_source(const int *lo, const int *hi, const double *up, const
                  A(f3, i, j, k, n) * A(a3, i, j, k, 0) - A(f3, i, j, k + 1, n) * A(a3, i, j, k + 1, 0)) *
                     volinv;
             if (n == UEINT) {
-                double pdivu =
+                real pdivu =
                     HALF * (A(qx, i + 1, j, k, GDPRES) + A(qx, i, j, k, GDPRES)) *
                         (A(qx, i + 1, j, k, GDU) - A(qx, i, j, k, GDU)) / dx[0] +
                     HALF * (A(qy, i, j + 1, k, GDPRES) + A(qy, i, j, k, GDPRES)) *
@@ -240,13 +240,13 @@ void orc_fill_hydro_source(const int *lo, const int *hi, const double *up, const
 
 /* ------------------------------------------------------------------ */
 /* Source/ppm.F90:11-128 */
-void orc_uflatten(int i, int j, int k, const double *qp_, const int *q_lo, const int *q_hi,
-                  double *flatn)
+void orc_uflatten(int i, int j, int k, const real *qp_, const int *q_lo, const int *q_hi,
+                  real *flatn)
 {
     arr_t q = mk(qp_, q_lo, q_hi);
-    const double shktst = 0.33, zcut1 = 0.75, zcut2 = 0.85;
-    const double dzcut = ONE / (zcut2 - zcut1);
-    double pl, pr, dp, dp2, du, z, z2, chi, chi2, denom, denominv, zeta, tst, tmp;
+    const real shktst = 0.33, zcut1 = 0.75, zcut2 = 0.85;
+    const real dzcut = ONE / (zcut2 - zcut1);
+    real pl, pr, dp, dp2, du, z, z2, chi, chi2, denom, denominv, zeta, tst, tmp;
     int ishft;
     *flatn = ONE;
     for (int idir = 1; idir <= 3; ++idir) {
@@ -306,10 +306,10 @@ void orc_uflatten(int i, int j, int k, const double *qp_, const int *q_lo, const
 }
 
 /* Source/ppm.F90:132-236; s5[0..4] = s(-2..2) */
-void orc_ppm_reconstruct(const double *s5, double flatn, double *smo, double *spo)
+void orc_ppm_reconstruct(const real *s5, real flatn, real *smo, real *spo)
 {
-    const double *s = s5 + 2;
-    double dsl, dsr, dsc, dsvl_l, dsvl_r, sm, sp;
+    const real *s = s5 + 2;
+    real dsl, dsr, dsc, dsvl_l, dsvl_r, sm, sp;
 
     dsl = TWO * (s[-1] - s[-2]);
     dsr = TWO * (s[0] - s[-1]);
@@ -359,12 +359,12 @@ void orc_ppm_reconstruct(const double *s5, double flatn, double *smo, double *sp
 }
 
 /* Source/ppm.F90:240-305 */
-void orc_ppm_int_profile(double sm, double sp, double sc, double u, double c, double dtdx,
-                         double *Ip, double *Im)
+void orc_ppm_int_profile(real sm, real sp, real sc, real u, real c, real dtdx,
+                         real *Ip, real *Im)
 {
-    double speed, sigma, s6;
+    real speed, sigma, s6;
     s6 = SIX * sc - THREE * (sm + sp);
-    const double sp3[3] = {u - c, u, u + c};
+    const real sp3[3] = {u - c, u, u + c};
     for (int w = 0; w < 3; ++w) {
         speed = sp3[w];
         sigma = fabs(speed) * dtdx;
@@ -380,15 +380,15 @@ void orc_ppm_int_profile(double sm, double sp, double sc, double u, double c, do
 
 /* Source/ppm.F90:309-743 */
 void orc_trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi, int idir,
-                   const double *qp_, const int *q_lo, const int *q_hi, const double *qap,
-                   const int *qa_lo, const int *qa_hi, double *qmp, const int *qm_lo,
-                   const int *qm_hi, double *qpp, const int *qp_lo, const int *qp_hi,
-                   const int *domlo, const int *domhi, const double *dx, double dt)
+                   const real *qp_, const int *q_lo, const int *q_hi, const real *qap,
+                   const int *qa_lo, const int *qa_hi, real *qmp, const int *qm_lo,
+                   const int *qm_hi, real *qpp, const int *qp_lo, const int *qp_hi,
+                   const int *domlo, const int *domhi, const real *dx, real dt)
 {
     (void)domlo; (void)domhi;
     arr_t q = mk(qp_, q_lo, q_hi), qaux = mk(qap, qa_lo, qa_hi), qm = mk(qmp, qm_lo, qm_hi),
           qp = mk(qpp, qp_lo, qp_hi);
-    const double dtdx = dt / dx[idir - 1];
+    const real dtdx = dt / dx[idir - 1];
     int QUN, QUT, QUTT;
     if (idir == 1) { QUN = QU; QUT = QV; QUTT = QW; }
     else if (idir == 2) { QUN = QV; QUT = QW; QUTT = QU; }
@@ -397,19 +397,19 @@ void orc_trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi,
 
     LOOP3(lo, hi)
     {
-        double Ip[3][ORC_QVAR], Im[3][ORC_QVAR], Ip_gc[3], Im_gc[3], Ip_sp[3], Im_sp[3];
-        double s[5], sm, sp, flatn;
-        double rho = A(q, i, j, k, QRHO);
+        real Ip[3][ORC_QVAR], Im[3][ORC_QVAR], Ip_gc[3], Im_gc[3], Ip_sp[3], Im_sp[3];
+        real s[5], sm, sp, flatn;
+        real rho = A(q, i, j, k, QRHO);
         (void)rho;
-        double cc = A(qaux, i, j, k, QC);
-        double un = A(q, i, j, k, QUN);
+        real cc = A(qaux, i, j, k, QC);
+        real un = A(q, i, j, k, QUN);
         orc_uflatten(i, j, k, qp_, q_lo, q_hi, &flatn);
 
         for (int n = 0; n <= QTEMP; ++n) {
             if (n == QGAME || n == QTEMP) continue; /* reconstruct_state, :419-421 */
             for (int m = -2; m <= 2; ++m) s[m + 2] = A(q, i + m * di, j + m * dj, k + m * dk, n);
             orc_ppm_reconstruct(s, flatn, &sm, &sp);
-            double ip3[3], im3[3];
+            real ip3[3], im3[3];
             orc_ppm_int_profile(sm, sp, s[2], un, cc, dtdx, ip3, im3);
             for (int w = 0; w < 3; ++w) { Ip[w][n] = ip3[w]; Im[w][n] = im3[w]; }
         }
@@ -432,26 +432,26 @@ void orc_trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi,
         }
 
         if (do_p) { /* :529-625 */
-            double rho_ref = Im[0][QRHO], un_ref = Im[0][QUN], p_ref = Im[0][QPRES];
-            double rhoe_g_ref = Im[0][QREINT], gam_g_ref = Im_gc[0];
+            real rho_ref = Im[0][QRHO], un_ref = Im[0][QUN], p_ref = Im[0][QPRES];
+            real rhoe_g_ref = Im[0][QREINT], gam_g_ref = Im_gc[0];
             rho_ref = fmax(rho_ref, small_dens);
-            double rho_ref_inv = ONE / rho_ref;
+            real rho_ref_inv = ONE / rho_ref;
             p_ref = fmax(p_ref, small_pres);
-            double csq_ref = gam_g_ref * p_ref * rho_ref_inv;
-            double cc_ref = sqrt(csq_ref);
-            double cc_ref_inv = ONE / cc_ref;
-            double h_g_ref = (p_ref + rhoe_g_ref) * rho_ref_inv;
-            double dum = un_ref - Im[0][QUN];
-            double dptotm = p_ref - Im[0][QPRES];
-            double drho = rho_ref - Im[1][QRHO];
-            double dptot = p_ref - Im[1][QPRES];
-            double drhoe_g = rhoe_g_ref - Im[1][QREINT];
-            double dup = un_ref - Im[2][QUN];
-            double dptotp = p_ref - Im[2][QPRES];
-            double alpham = HALF * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
-            double alphap = HALF * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
-            double alpha0r = drho - dptot / csq_ref;
-            double alpha0e_g = drhoe_g - dptot * h_g_ref / csq_ref;
+            real csq_ref = gam_g_ref * p_ref * rho_ref_inv;
+            real cc_ref = sqrt(csq_ref);
+            real cc_ref_inv = ONE / cc_ref;
+            real h_g_ref = (p_ref + rhoe_g_ref) * rho_ref_inv;
+            real dum = un_ref - Im[0][QUN];
+            real dptotm = p_ref - Im[0][QPRES];
+            real drho = rho_ref - Im[1][QRHO];
+            real dptot = p_ref - Im[1][QPRES];
+            real drhoe_g = rhoe_g_ref - Im[1][QREINT];
+            real dup = un_ref - Im[2][QUN];
+            real dptotp = p_ref - Im[2][QPRES];
+            real alpham = HALF * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+            real alphap = HALF * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+            real alpha0r = drho - dptot / csq_ref;
+            real alpha0e_g = drhoe_g - dptot * h_g_ref / csq_ref;
             alpham = (un - cc > ZERO) ? ZERO : -alpham;
             alphap = (un + cc > ZERO) ? ZERO : -alphap;
             alpha0r = (un > ZERO) ? ZERO : -alpha0r;
@@ -464,26 +464,26 @@ void orc_trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi,
             A(qp, i, j, k, QUTT) = Im[1][QUTT];
         }
         if (do_m) { /* :631-737 */
-            double rho_ref = Ip[2][QRHO], un_ref = Ip[2][QUN], p_ref = Ip[2][QPRES];
-            double rhoe_g_ref = Ip[2][QREINT], gam_g_ref = Ip_gc[2];
+            real rho_ref = Ip[2][QRHO], un_ref = Ip[2][QUN], p_ref = Ip[2][QPRES];
+            real rhoe_g_ref = Ip[2][QREINT], gam_g_ref = Ip_gc[2];
             rho_ref = fmax(rho_ref, small_dens);
-            double rho_ref_inv = ONE / rho_ref;
+            real rho_ref_inv = ONE / rho_ref;
             p_ref = fmax(p_ref, small_pres);
-            double csq_ref = gam_g_ref * p_ref * rho_ref_inv;
-            double cc_ref = sqrt(csq_ref);
-            double cc_ref_inv = ONE / cc_ref;
-            double h_g_ref = (p_ref + rhoe_g_ref) * rho_ref_inv;
-            double dum = un_ref - Ip[0][QUN];
-            double dptotm = p_ref - Ip[0][QPRES];
-            double drho = rho_ref - Ip[1][QRHO];
-            double dptot = p_ref - Ip[1][QPRES];
-            double drhoe_g = rhoe_g_ref - Ip[1][QREINT];
-            double dup = un_ref - Ip[2][QUN];
-            double dptotp = p_ref - Ip[2][QPRES];
-            double alpham = HALF * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
-            double alphap = HALF * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
-            double alpha0r = drho - dptot / csq_ref;
-            double alpha0e_g = drhoe_g - dptot * h_g_ref / csq_ref;
+            real csq_ref = gam_g_ref * p_ref * rho_ref_inv;
+            real cc_ref = sqrt(csq_ref);
+            real cc_ref_inv = ONE / cc_ref;
+            real h_g_ref = (p_ref + rhoe_g_ref) * rho_ref_inv;
+            real dum = un_ref - Ip[0][QUN];
+            real dptotm = p_ref - Ip[0][QPRES];
+            real drho = rho_ref - Ip[1][QRHO];
+            real dptot = p_ref - Ip[1][QPRES];
+            real drhoe_g = rhoe_g_ref - Ip[1][QREINT];
+            real dup = un_ref - Ip[2][QUN];
+            real dptotp = p_ref - Ip[2][QPRES];
+            real alpham = HALF * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+            real alphap = HALF * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+            real alpha0r = drho - dptot / csq_ref;
+            real alpha0e_g = drhoe_g - dptot * h_g_ref / csq_ref;
             alpham = (un - cc > ZERO) ? -alpham : ZERO;
             alphap = (un + cc > ZERO) ? -alphap : ZERO;
             alpha0r = (un > ZERO) ? -alpha0r : ZERO;
@@ -501,11 +501,11 @@ void orc_trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi,
 
 /* ------------------------------------------------------------------ */
 /* Source/riemann.F90:10-331 */
-void orc_compute_flux(const int *lo, const int *hi, const double *qlp, const int *ql_lo,
-                      const int *ql_hi, const double *qrp, const int *qr_lo, const int *qr_hi,
-                      double *fp, const int *f_lo, const int *f_hi, double *qip, const int *qi_lo,
-                      const int *qi_hi, double *qgp, const int *qg_lo, const int *qg_hi,
-                      const double *qap, const int *qa_lo, const int *qa_hi, int idir)
+void orc_compute_flux(const int *lo, const int *hi, const real *qlp, const int *ql_lo,
+                      const int *ql_hi, const real *qrp, const int *qr_lo, const int *qr_hi,
+                      real *fp, const int *f_lo, const int *f_hi, real *qip, const int *qi_lo,
+                      const int *qi_hi, real *qgp, const int *qg_lo, const int *qg_hi,
+                      const real *qap, const int *qa_lo, const int *qa_hi, int idir)
 {
     arr_t ql = mk(qlp, ql_lo, ql_hi), qr = mk(qrp, qr_lo, qr_hi), flx = mk(fp, f_lo, f_hi),
           qint = mk(qip, qi_lo, qi_hi), qgdnv = mk(qgp, qg_lo, qg_hi), qaux = mk(qap, qa_lo, qa_hi);
@@ -517,70 +517,70 @@ void orc_compute_flux(const int *lo, const int *hi, const double *qlp, const int
 
     LOOP3(lo, hi)
     {
-        double rl = fmax(A(ql, i, j, k, QRHO), small_dens);
-        double ul = A(ql, i, j, k, iu), v1l = A(ql, i, j, k, iv1), v2l = A(ql, i, j, k, iv2);
-        double pl = fmax(A(ql, i, j, k, QPRES), small_pres);
-        double rel = A(ql, i, j, k, QREINT);
-        double rr = fmax(A(qr, i, j, k, QRHO), small_dens);
-        double ur = A(qr, i, j, k, iu), v1r = A(qr, i, j, k, iv1), v2r = A(qr, i, j, k, iv2);
-        double pr = fmax(A(qr, i, j, k, QPRES), small_pres);
-        double rer = A(qr, i, j, k, QREINT);
+        real rl = fmax(A(ql, i, j, k, QRHO), small_dens);
+        real ul = A(ql, i, j, k, iu), v1l = A(ql, i, j, k, iv1), v2l = A(ql, i, j, k, iv2);
+        real pl = fmax(A(ql, i, j, k, QPRES), small_pres);
+        real rel = A(ql, i, j, k, QREINT);
+        real rr = fmax(A(qr, i, j, k, QRHO), small_dens);
+        real ur = A(qr, i, j, k, iu), v1r = A(qr, i, j, k, iv1), v2r = A(qr, i, j, k, iv2);
+        real pr = fmax(A(qr, i, j, k, QPRES), small_pres);
+        real rer = A(qr, i, j, k, QREINT);
 
-        double cR = A(qaux, i, j, k, QC), cL = A(qaux, i - di, j - dj, k - dk, QC);
-        double csmall = fmax(small, small * fmax(cR, cL));
-        double cavg = HALF * (cR + cL);
-        double gamcl = A(qaux, i - di, j - dj, k - dk, QGAMC);
-        double gamcr = A(qaux, i, j, k, QGAMC);
+        real cR = A(qaux, i, j, k, QC), cL = A(qaux, i - di, j - dj, k - dk, QC);
+        real csmall = fmax(small, small * fmax(cR, cL));
+        real cavg = HALF * (cR + cL);
+        real gamcl = A(qaux, i - di, j - dj, k - dk, QGAMC);
+        real gamcr = A(qaux, i, j, k, QGAMC);
 
-        double wsmall = small_dens * csmall;
-        double wl = fmax(wsmall, sqrt(fabs(gamcl * pl * rl)));
-        double wr = fmax(wsmall, sqrt(fabs(gamcr * pr * rr)));
-        double wwinv = ONE / (wl + wr);
-        double pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) * wwinv;
-        double ustar = ((wl * ul + wr * ur) + (pl - pr)) * wwinv;
+        real wsmall = small_dens * csmall;
+        real wl = fmax(wsmall, sqrt(fabs(gamcl * pl * rl)));
+        real wr = fmax(wsmall, sqrt(fabs(gamcr * pr * rr)));
+        real wwinv = ONE / (wl + wr);
+        real pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) * wwinv;
+        real ustar = ((wl * ul + wr * ur) + (pl - pr)) * wwinv;
         pstar = fmax(pstar, small_pres);
         if (fabs(ustar) < smallu * HALF * (fabs(ul) + fabs(ur))) ustar = ZERO;
 
-        double sgnm = fsign1(ustar);
+        real sgnm = fsign1(ustar);
         if (ustar == ZERO) sgnm = ZERO;
-        double fpl = HALF * (ONE + sgnm);
-        double fmr = HALF * (ONE - sgnm);
-        double ro = fpl * rl + fmr * rr;
-        double uo = fpl * ul + fmr * ur;
-        double po = fpl * pl + fmr * pr;
-        double reo = fpl * rel + fmr * rer;
-        double gamco = fpl * gamcl + fmr * gamcr;
+        real fpl = HALF * (ONE + sgnm);
+        real fmr = HALF * (ONE - sgnm);
+        real ro = fpl * rl + fmr * rr;
+        real uo = fpl * ul + fmr * ur;
+        real po = fpl * pl + fmr * pr;
+        real reo = fpl * rel + fmr * rer;
+        real gamco = fpl * gamcl + fmr * gamcr;
         ro = fmax(small_dens, ro);
-        double roinv = ONE / ro;
-        double co = sqrt(fabs(gamco * po * roinv));
+        real roinv = ONE / ro;
+        real co = sqrt(fabs(gamco * po * roinv));
         co = fmax(csmall, co);
-        double co2inv = ONE / (co * co);
+        real co2inv = ONE / (co * co);
         A(qint, i, j, k, iv1) = fpl * v1l + fmr * v1r;
         A(qint, i, j, k, iv2) = fpl * v2l + fmr * v2r;
 
-        double drho = (pstar - po) * co2inv;
-        double rstar = ro + drho;
+        real drho = (pstar - po) * co2inv;
+        real rstar = ro + drho;
         rstar = fmax(small_dens, rstar);
-        double rstarinv = ONE / rstar;
-        double entho = (reo + po) * roinv * co2inv;
-        double estar = reo + (pstar - po) * entho;
-        double cstar = sqrt(fabs(gamco * pstar * rstarinv));
+        real rstarinv = ONE / rstar;
+        real entho = (reo + po) * roinv * co2inv;
+        real estar = reo + (pstar - po) * entho;
+        real cstar = sqrt(fabs(gamco * pstar * rstarinv));
         cstar = fmax(cstar, csmall);
 
         sgnm = fsign1(ustar);
-        double spout = co - sgnm * uo;
-        double spin = cstar - sgnm * ustar;
-        double ushock = HALF * (spin + spout);
+        real spout = co - sgnm * uo;
+        real spin = cstar - sgnm * ustar;
+        real ushock = HALF * (spin + spout);
         if (pstar - po > ZERO) { spin = ushock; spout = ushock; }
-        double scr = (spout - spin == ZERO) ? small * cavg : spout - spin;
-        double scrinv = ONE / scr;
-        double frac = (ONE + (spout + spin) * scrinv) * HALF;
+        real scr = (spout - spin == ZERO) ? small * cavg : spout - spin;
+        real scrinv = ONE / scr;
+        real frac = (ONE + (spout + spin) * scrinv) * HALF;
         frac = fmax(ZERO, fmin(ONE, frac));
 
         A(qint, i, j, k, QRHO) = frac * rstar + (ONE - frac) * ro;
         A(qint, i, j, k, iu) = frac * ustar + (ONE - frac) * uo;
         A(qint, i, j, k, QPRES) = frac * pstar + (ONE - frac) * po;
-        double regdnv = frac * estar + (ONE - frac) * reo;
+        real regdnv = frac * estar + (ONE - frac) * reo;
         if (spout < ZERO) {
             A(qint, i, j, k, QRHO) = ro;
             A(qint, i, j, k, iu) = uo;
@@ -593,7 +593,7 @@ void orc_compute_flux(const int *lo, const int *hi, const double *qlp, const int
             A(qint, i, j, k, QPRES) = pstar;
             regdnv = estar;
         }
-        double regdnvinv = ONE / regdnv;
+        real regdnvinv = ONE / regdnv;
         A(qint, i, j, k, QGAME) = A(qint, i, j, k, QPRES) * regdnvinv + ONE;
         A(qint, i, j, k, QPRES) = fmax(A(qint, i, j, k, QPRES), small_pres);
         A(qint, i, j, k, QREINT) = regdnv;
@@ -609,13 +609,13 @@ void orc_compute_flux(const int *lo, const int *hi, const double *qlp, const int
         A(qgdnv, i, j, k, GDPRES) = A(qint, i, j, k, QPRES);
         A(qgdnv, i, j, k, GDGAME) = A(qint, i, j, k, QGAME);
 
-        double u_adv = A(qint, i, j, k, iu);
-        double rhoeint = A(qint, i, j, k, QREINT);
+        real u_adv = A(qint, i, j, k, iu);
+        real rhoeint = A(qint, i, j, k, QREINT);
         A(flx, i, j, k, URHO) = A(qint, i, j, k, QRHO) * u_adv;
         A(flx, i, j, k, im1) = A(flx, i, j, k, URHO) * A(qint, i, j, k, iu);
         A(flx, i, j, k, im2) = A(flx, i, j, k, URHO) * A(qint, i, j, k, iv1);
         A(flx, i, j, k, im3) = A(flx, i, j, k, URHO) * A(qint, i, j, k, iv2);
-        double rhoetot = rhoeint + HALF * A(qint, i, j, k, QRHO) *
+        real rhoetot = rhoeint + HALF * A(qint, i, j, k, QRHO) *
                                        (A(qint, i, j, k, iu) * A(qint, i, j, k, iu) +
                                         A(qint, i, j, k, iv1) * A(qint, i, j, k, iv1) +
                                         A(qint, i, j, k, iv2) * A(qint, i, j, k, iv2));
@@ -630,13 +630,13 @@ void orc_compute_flux(const int *lo, const int *hi, const double *qlp, const int
 /* ------------------------------------------------------------------ */
 /* Source/trans.F90:14-243.  QGAME/QTEMP of the outputs are left untouched
    (the reference copies an uninitialised private array there). */
-void orc_trans1(const int *lo, const int *hi, int idir1, int idir2, const double *q2mp,
-                const int *q2m_lo, const int *q2m_hi, double *q2mop, const int *q2mo_lo,
-                const int *q2mo_hi, const double *q2pp, const int *q2p_lo, const int *q2p_hi,
-                double *q2pop, const int *q2po_lo, const int *q2po_hi, const double *qap,
-                const int *qa_lo, const int *qa_hi, const double *f1p, const int *f1_lo,
-                const int *f1_hi, const double *q1p, const int *q1_lo, const int *q1_hi,
-                double cdtdx)
+void orc_trans1(const int *lo, const int *hi, int idir1, int idir2, const real *q2mp,
+                const int *q2m_lo, const int *q2m_hi, real *q2mop, const int *q2mo_lo,
+                const int *q2mo_hi, const real *q2pp, const int *q2p_lo, const int *q2p_hi,
+                real *q2pop, const int *q2po_lo, const int *q2po_hi, const real *qap,
+                const int *qa_lo, const int *qa_hi, const real *f1p, const int *f1_lo,
+                const int *f1_hi, const real *q1p, const int *q1_lo, const int *q1_hi,
+                real cdtdx)
 {
     arr_t q2m = mk(q2mp, q2m_lo, q2m_hi), q2mo = mk(q2mop, q2mo_lo, q2mo_hi),
           q2p = mk(q2pp, q2p_lo, q2p_hi), q2po = mk(q2pop, q2po_lo, q2po_hi),
@@ -652,37 +652,37 @@ void orc_trans1(const int *lo, const int *hi, int idir1, int idir2, const double
             else if (idir2 == 2) { jl += d; jr += d; }
             else { kl += d; kr += d; }
 
-            double lq2[ORC_QVAR], lq2o[ORC_QVAR];
+            real lq2[ORC_QVAR], lq2o[ORC_QVAR];
             int have_game = 0;
             for (int n = 0; n < ORC_QVAR; ++n)
                 lq2[n] = (d == -1) ? A(q2m, i, j, k, n) : A(q2p, i, j, k, n);
 
             for (int ispec = 0; ispec < ORC_NSPEC; ++ispec) {
                 int n = UFS + ispec, nqp = QFS + ispec;
-                double rrnew = lq2[QRHO] - cdtdx * (A(f1, ir, jr, kr, URHO) - A(f1, il, jl, kl, URHO));
-                double compu = lq2[QRHO] * lq2[nqp] - cdtdx * (A(f1, ir, jr, kr, n) - A(f1, il, jl, kl, n));
+                real rrnew = lq2[QRHO] - cdtdx * (A(f1, ir, jr, kr, URHO) - A(f1, il, jl, kl, URHO));
+                real compu = lq2[QRHO] * lq2[nqp] - cdtdx * (A(f1, ir, jr, kr, n) - A(f1, il, jl, kl, n));
                 lq2o[nqp] = compu / rrnew;
             }
 
-            double pgp = A(q1, ir, jr, kr, GDPRES), pgm = A(q1, il, jl, kl, GDPRES);
-            double ugp = A(q1, ir, jr, kr, GDU + idir1 - 1), ugm = A(q1, il, jl, kl, GDU + idir1 - 1);
-            double dup = pgp * ugp - pgm * ugm;
-            double du = ugp - ugm;
-            double pav = HALF * (pgp + pgm);
-            double gamc = A(qaux, il, jl, kl, QGAMC);
+            real pgp = A(q1, ir, jr, kr, GDPRES), pgm = A(q1, il, jl, kl, GDPRES);
+            real ugp = A(q1, ir, jr, kr, GDU + idir1 - 1), ugm = A(q1, il, jl, kl, GDU + idir1 - 1);
+            real dup = pgp * ugp - pgm * ugm;
+            real du = ugp - ugm;
+            real pav = HALF * (pgp + pgm);
+            real gamc = A(qaux, il, jl, kl, QGAMC);
 
-            double rrl2 = lq2[QRHO];
-            double rul2 = rrl2 * lq2[QU];
-            double rvl2 = rrl2 * lq2[QV];
-            double rwl2 = rrl2 * lq2[QW];
-            double ekenl2 = HALF * rrl2 * (lq2[QU] * lq2[QU] + lq2[QV] * lq2[QV] + lq2[QW] * lq2[QW]);
-            double rel2 = lq2[QREINT] + ekenl2;
+            real rrl2 = lq2[QRHO];
+            real rul2 = rrl2 * lq2[QU];
+            real rvl2 = rrl2 * lq2[QV];
+            real rwl2 = rrl2 * lq2[QW];
+            real ekenl2 = HALF * rrl2 * (lq2[QU] * lq2[QU] + lq2[QV] * lq2[QV] + lq2[QW] * lq2[QW]);
+            real rel2 = lq2[QREINT] + ekenl2;
 
-            double rrnewl2 = rrl2 - cdtdx * (A(f1, ir, jr, kr, URHO) - A(f1, il, jl, kl, URHO));
-            double runewl2 = rul2 - cdtdx * (A(f1, ir, jr, kr, UMX) - A(f1, il, jl, kl, UMX));
-            double rvnewl2 = rvl2 - cdtdx * (A(f1, ir, jr, kr, UMY) - A(f1, il, jl, kl, UMY));
-            double rwnewl2 = rwl2 - cdtdx * (A(f1, ir, jr, kr, UMZ) - A(f1, il, jl, kl, UMZ));
-            double renewl2 = rel2 - cdtdx * (A(f1, ir, jr, kr, UEDEN) - A(f1, il, jl, kl, UEDEN));
+            real rrnewl2 = rrl2 - cdtdx * (A(f1, ir, jr, kr, URHO) - A(f1, il, jl, kl, URHO));
+            real runewl2 = rul2 - cdtdx * (A(f1, ir, jr, kr, UMX) - A(f1, il, jl, kl, UMX));
+            real rvnewl2 = rvl2 - cdtdx * (A(f1, ir, jr, kr, UMY) - A(f1, il, jl, kl, UMY));
+            real rwnewl2 = rwl2 - cdtdx * (A(f1, ir, jr, kr, UMZ) - A(f1, il, jl, kl, UMZ));
+            real renewl2 = rel2 - cdtdx * (A(f1, ir, jr, kr, UEDEN) - A(f1, il, jl, kl, UEDEN));
 
             int reset_state = 0;
             if (rrnewl2 < ZERO) {
@@ -694,14 +694,14 @@ void orc_trans1(const int *lo, const int *hi, int idir1, int idir2, const double
                 reset_state = 1;
             }
             lq2o[QRHO] = rrnewl2;
-            double rhoinv = ONE / rrnewl2;
+            real rhoinv = ONE / rrnewl2;
             lq2o[QU] = runewl2 * rhoinv;
             lq2o[QV] = rvnewl2 * rhoinv;
             lq2o[QW] = rwnewl2 * rhoinv;
-            double rhoekenl2 = HALF * (runewl2 * runewl2 + rvnewl2 * rvnewl2 + rwnewl2 * rwnewl2) * rhoinv;
+            real rhoekenl2 = HALF * (runewl2 * runewl2 + rvnewl2 * rvnewl2 + rwnewl2 * rwnewl2) * rhoinv;
             lq2o[QREINT] = renewl2 - rhoekenl2;
             if (!reset_state) {
-                double pnewl2 = lq2[QPRES] - cdtdx * (dup + pav * du * (gamc - ONE));
+                real pnewl2 = lq2[QPRES] - cdtdx * (dup + pav * du * (gamc - ONE));
                 lq2o[QPRES] = fmax(pnewl2, small_pres);
             } else {
                 lq2o[QPRES] = lq2[QPRES];
@@ -718,14 +718,14 @@ void orc_trans1(const int *lo, const int *hi, int idir1, int idir2, const double
 }
 
 /* Source/trans.F90:250-505 */
-void orc_trans2(const int *lo, const int *hi, int idir1, int idir2, int idir3, const double *qm1p,
-                const int *qm1_lo, const int *qm1_hi, double *qm1op, const int *qm1o_lo,
-                const int *qm1o_hi, const double *qp1p, const int *qp1_lo, const int *qp1_hi,
-                double *qp1op, const int *qp1o_lo, const int *qp1o_hi, const double *qap,
-                const int *qa_lo, const int *qa_hi, const double *f2p, const int *f2_lo,
-                const int *f2_hi, const double *f3p, const int *f3_lo, const int *f3_hi,
-                const double *q2p, const int *q2_lo, const int *q2_hi, const double *q3p,
-                const int *q3_lo, const int *q3_hi, double cdtdx1, double cdtdx2, double cdtdx3)
+void orc_trans2(const int *lo, const int *hi, int idir1, int idir2, int idir3, const real *qm1p,
+                const int *qm1_lo, const int *qm1_hi, real *qm1op, const int *qm1o_lo,
+                const int *qm1o_hi, const real *qp1p, const int *qp1_lo, const int *qp1_hi,
+                real *qp1op, const int *qp1o_lo, const int *qp1o_hi, const real *qap,
+                const int *qa_lo, const int *qa_hi, const real *f2p, const int *f2_lo,
+                const int *f2_hi, const real *f3p, const int *f3_lo, const int *f3_hi,
+                const real *q2p, const int *q2_lo, const int *q2_hi, const real *q3p,
+                const int *q3_lo, const int *q3_hi, real cdtdx1, real cdtdx2, real cdtdx3)
 {
     (void)cdtdx1;
     arr_t qm1 = mk(qm1p, qm1_lo, qm1_hi), qm1o = mk(qm1op, qm1o_lo, qm1o_hi),
@@ -750,52 +750,52 @@ void orc_trans2(const int *lo, const int *hi, int idir1, int idir2, int idir3, c
                 ir3 = i; jr3 = j + 1; kr3 = k + d;
                 kl1 = k + d; kl2 = k + d; kl3 = k + d;
             }
-            double lq[ORC_QVAR], lqo[ORC_QVAR];
+            real lq[ORC_QVAR], lqo[ORC_QVAR];
             int have_game = 0;
             for (int n = 0; n < ORC_QVAR; ++n)
                 lq[n] = (d == -1) ? A(qm1, i, j, k, n) : A(qp1, i, j, k, n);
 
             for (int ispec = 0; ispec < ORC_NSPEC; ++ispec) {
                 int n = UFS + ispec, nqp = QFS + ispec;
-                double rrr = lq[QRHO];
-                double compr = rrr * lq[nqp];
-                double rrnewr = rrr - cdtdx2 * (A(f2, ir2, jr2, kr2, URHO) - A(f2, il2, jl2, kl2, URHO)) -
+                real rrr = lq[QRHO];
+                real compr = rrr * lq[nqp];
+                real rrnewr = rrr - cdtdx2 * (A(f2, ir2, jr2, kr2, URHO) - A(f2, il2, jl2, kl2, URHO)) -
                                 cdtdx3 * (A(f3, ir3, jr3, kr3, URHO) - A(f3, il3, jl3, kl3, URHO));
-                double compnr = compr - cdtdx2 * (A(f2, ir2, jr2, kr2, n) - A(f2, il2, jl2, kl2, n)) -
+                real compnr = compr - cdtdx2 * (A(f2, ir2, jr2, kr2, n) - A(f2, il2, jl2, kl2, n)) -
                                 cdtdx3 * (A(f3, ir3, jr3, kr3, n) - A(f3, il3, jl3, kl3, n));
                 lqo[nqp] = compnr / rrnewr;
             }
 
-            double pg2p = A(q2, ir2, jr2, kr2, GDPRES), pg2m = A(q2, il2, jl2, kl2, GDPRES);
-            double ug2p = A(q2, ir2, jr2, kr2, GDU + idir2 - 1), ug2m = A(q2, il2, jl2, kl2, GDU + idir2 - 1);
-            double du2p = pg2p * ug2p - pg2m * ug2m;
-            double p2av = HALF * (pg2p + pg2m);
-            double du2 = ug2p - ug2m;
-            double p2new = cdtdx2 * (du2p + p2av * du2 * (A(qaux, il1, jl1, kl1, QGAMC) - ONE));
+            real pg2p = A(q2, ir2, jr2, kr2, GDPRES), pg2m = A(q2, il2, jl2, kl2, GDPRES);
+            real ug2p = A(q2, ir2, jr2, kr2, GDU + idir2 - 1), ug2m = A(q2, il2, jl2, kl2, GDU + idir2 - 1);
+            real du2p = pg2p * ug2p - pg2m * ug2m;
+            real p2av = HALF * (pg2p + pg2m);
+            real du2 = ug2p - ug2m;
+            real p2new = cdtdx2 * (du2p + p2av * du2 * (A(qaux, il1, jl1, kl1, QGAMC) - ONE));
 
-            double pg3p = A(q3, ir3, jr3, kr3, GDPRES), pg3m = A(q3, il3, jl3, kl3, GDPRES);
-            double ug3p = A(q3, ir3, jr3, kr3, GDU + idir3 - 1), ug3m = A(q3, il3, jl3, kl3, GDU + idir3 - 1);
-            double du3p = pg3p * ug3p - pg3m * ug3m;
-            double p3av = HALF * (pg3p + pg3m);
-            double du3 = ug3p - ug3m;
-            double p3new = cdtdx3 * (du3p + p3av * du3 * (A(qaux, il1, jl1, kl1, QGAMC) - ONE));
+            real pg3p = A(q3, ir3, jr3, kr3, GDPRES), pg3m = A(q3, il3, jl3, kl3, GDPRES);
+            real ug3p = A(q3, ir3, jr3, kr3, GDU + idir3 - 1), ug3m = A(q3, il3, jl3, kl3, GDU + idir3 - 1);
+            real du3p = pg3p * ug3p - pg3m * ug3m;
+            real p3av = HALF * (pg3p + pg3m);
+            real du3 = ug3p - ug3m;
+            real p3new = cdtdx3 * (du3p + p3av * du3 * (A(qaux, il1, jl1, kl1, QGAMC) - ONE));
 
-            double rrr = lq[QRHO];
-            double rur = rrr * lq[QU];
-            double rvr = rrr * lq[QV];
-            double rwr = rrr * lq[QW];
-            double ekenr = HALF * rrr * (lq[QU] * lq[QU] + lq[QV] * lq[QV] + lq[QW] * lq[QW]);
-            double rer = lq[QREINT] + ekenr;
+            real rrr = lq[QRHO];
+            real rur = rrr * lq[QU];
+            real rvr = rrr * lq[QV];
+            real rwr = rrr * lq[QW];
+            real ekenr = HALF * rrr * (lq[QU] * lq[QU] + lq[QV] * lq[QV] + lq[QW] * lq[QW]);
+            real rer = lq[QREINT] + ekenr;
 
-            double rrnewr = rrr - cdtdx2 * (A(f2, ir2, jr2, kr2, URHO) - A(f2, il2, jl2, kl2, URHO)) -
+            real rrnewr = rrr - cdtdx2 * (A(f2, ir2, jr2, kr2, URHO) - A(f2, il2, jl2, kl2, URHO)) -
                             cdtdx3 * (A(f3, ir3, jr3, kr3, URHO) - A(f3, il3, jl3, kl3, URHO));
-            double runewr = rur - cdtdx2 * (A(f2, ir2, jr2, kr2, UMX) - A(f2, il2, jl2, kl2, UMX)) -
+            real runewr = rur - cdtdx2 * (A(f2, ir2, jr2, kr2, UMX) - A(f2, il2, jl2, kl2, UMX)) -
                             cdtdx3 * (A(f3, ir3, jr3, kr3, UMX) - A(f3, il3, jl3, kl3, UMX));
-            double rvnewr = rvr - cdtdx2 * (A(f2, ir2, jr2, kr2, UMY) - A(f2, il2, jl2, kl2, UMY)) -
+            real rvnewr = rvr - cdtdx2 * (A(f2, ir2, jr2, kr2, UMY) - A(f2, il2, jl2, kl2, UMY)) -
                             cdtdx3 * (A(f3, ir3, jr3, kr3, UMY) - A(f3, il3, jl3, kl3, UMY));
-            double rwnewr = rwr - cdtdx2 * (A(f2, ir2, jr2, kr2, UMZ) - A(f2, il2, jl2, kl2, UMZ)) -
+            real rwnewr = rwr - cdtdx2 * (A(f2, ir2, jr2, kr2, UMZ) - A(f2, il2, jl2, kl2, UMZ)) -
                             cdtdx3 * (A(f3, ir3, jr3, kr3, UMZ) - A(f3, il3, jl3, kl3, UMZ));
-            double renewr = rer - cdtdx2 * (A(f2, ir2, jr2, kr2, UEDEN) - A(f2, il2, jl2, kl2, UEDEN)) -
+            real renewr = rer - cdtdx2 * (A(f2, ir2, jr2, kr2, UEDEN) - A(f2, il2, jl2, kl2, UEDEN)) -
                             cdtdx3 * (A(f3, ir3, jr3, kr3, UEDEN) - A(f3, il3, jl3, kl3, UEDEN));
 
             int reset_state = 0;
@@ -811,10 +811,10 @@ void orc_trans2(const int *lo, const int *hi, int idir1, int idir2, int idir3, c
             lqo[QU] = runewr / rrnewr;
             lqo[QV] = rvnewr / rrnewr;
             lqo[QW] = rwnewr / rrnewr;
-            double rhoekenr = HALF * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+            real rhoekenr = HALF * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
             lqo[QREINT] = renewr - rhoekenr;
             if (!reset_state) {
-                double pnewr = lq[QPRES] - p2new - p3new;
+                real pnewr = lq[QPRES] - p2new - p3new;
                 lqo[QPRES] = pnewr;
             } else {
                 lqo[QPRES] = lq[QPRES];
@@ -834,10 +834,10 @@ void orc_trans2(const int *lo, const int *hi, int idir1, int idir2, int idir3, c
 /* ------------------------------------------------------------------ */
 /* abar, zbar from species partial densities u(UFS..)*rhoInv, as written in
    Source/Castro_nd.F90:285-286 and Source/timestep.F90:49-50 */
-static void abar_zbar_rhoinv(const arr_t *u, int i, int j, int k, double rhoInv, double *abar,
-                             double *zbar)
+static void abar_zbar_rhoinv(const arr_t *u, int i, int j, int k, real rhoInv, real *abar,
+                             real *zbar)
 {
-    double s1 = 0.0, s2 = 0.0;
+    real s1 = 0.0, s2 = 0.0;
     for (int n = 0; n < ORC_NSPEC; ++n) s1 = s1 + A(*u, i, j, k, UFS + n) * aion_inv(n);
     *abar = ONE / (s1 * rhoInv);
     for (int n = 0; n < ORC_NSPEC; ++n) s2 = s2 + A(*u, i, j, k, UFS + n) * zion[n] * aion_inv(n);
@@ -845,32 +845,32 @@ static void abar_zbar_rhoinv(const arr_t *u, int i, int j, int k, double rhoInv,
 }
 
 /* Source/timestep.F90:12-72 (CPU branch of reduce_min, Source/reduction.F90:137-141) */
-void orc_estdt(const int *lo, const int *hi, const double *up, const int *u_lo, const int *u_hi,
-               const double *dx, double *dt)
+void orc_estdt(const int *lo, const int *hi, const real *up, const int *u_lo, const int *u_hi,
+               const real *dx, real *dt)
 {
     arr_t u = mk(up, u_lo, u_hi);
     LOOP3(lo, hi)
     {
-        double rhoInv = ONE / A(u, i, j, k, URHO);
+        real rhoInv = ONE / A(u, i, j, k, URHO);
         orc_eos_t st;
         st.rho = A(u, i, j, k, URHO);
         st.T = A(u, i, j, k, UTEMP);
         st.e = A(u, i, j, k, UEINT) * rhoInv;
         abar_zbar_rhoinv(&u, i, j, k, rhoInv, &st.abar, &st.zbar);
         orc_eos(EOS_INPUT_RE, &st);
-        double ux = A(u, i, j, k, UMX) * rhoInv;
-        double uy = A(u, i, j, k, UMY) * rhoInv;
-        double uz = A(u, i, j, k, UMZ) * rhoInv;
-        double c = st.cs;
-        double dt1 = dx[0] / (c + fabs(ux));
-        double dt2 = dx[1] / (c + fabs(uy));
-        double dt3 = dx[2] / (c + fabs(uz));
+        real ux = A(u, i, j, k, UMX) * rhoInv;
+        real uy = A(u, i, j, k, UMY) * rhoInv;
+        real uz = A(u, i, j, k, UMZ) * rhoInv;
+        real c = st.cs;
+        real dt1 = dx[0] / (c + fabs(ux));
+        real dt2 = dx[1] / (c + fabs(uy));
+        real dt3 = dx[2] / (c + fabs(uz));
         *dt = fmin(*dt, fmin(fmin(dt1, dt2), dt3));
     }
 }
 
 /* Source/Castro_nd.F90:69-125 */
-void orc_enforce_minimum_density(const int *lo, const int *hi, double *up, const int *u_lo,
+void orc_enforce_minimum_density(const int *lo, const int *hi, real *up, const int *u_lo,
                                  const int *u_hi)
 {
     arr_t u = mk(up, u_lo, u_hi);
@@ -884,7 +884,7 @@ void orc_enforce_minimum_density(const int *lo, const int *hi, double *up, const
             orc_eos_t st;
             st.rho = small_dens;
             st.T = small_temp;
-            double s1 = 0.0, s2 = 0.0;
+            real s1 = 0.0, s2 = 0.0;
             for (int n = 0; n < ORC_NSPEC; ++n) s1 = s1 + A(u, i, j, k, UFS + n) * aion_inv(n);
             st.abar = ONE / (s1 / small_dens);
             for (int n = 0; n < ORC_NSPEC; ++n) s2 = s2 + A(u, i, j, k, UFS + n) * zion[n] * aion_inv(n);
@@ -902,14 +902,14 @@ void orc_enforce_minimum_density(const int *lo, const int *hi, double *up, const
 }
 
 /* Source/Castro_nd.F90:129-160: stores X = (rho X)/sum, not rho X */
-void orc_normalize_species(const int *lo, const int *hi, double *up, const int *u_lo,
+void orc_normalize_species(const int *lo, const int *hi, real *up, const int *u_lo,
                            const int *u_hi)
 {
     arr_t u = mk(up, u_lo, u_hi);
     LOOP3(lo, hi)
     {
-        double rho = A(u, i, j, k, URHO);
-        double x[ORC_NSPEC], sum = 0.0;
+        real rho = A(u, i, j, k, URHO);
+        real x[ORC_NSPEC], sum = 0.0;
         for (int n = 0; n < ORC_NSPEC; ++n) x[n] = fmax(1.0e-30 * rho, fmin(rho, A(u, i, j, k, UFS + n)));
         for (int n = 0; n < ORC_NSPEC; ++n) sum = sum + x[n];
         for (int n = 0; n < ORC_NSPEC; ++n) A(u, i, j, k, UFS + n) = x[n] / sum;
@@ -917,17 +917,17 @@ void orc_normalize_species(const int *lo, const int *hi, double *up, const int *
 }
 
 /* Source/Castro_nd.F90:164-249 */
-void orc_reset_internal_e(const int *lo, const int *hi, double *up, const int *u_lo, const int *u_hi)
+void orc_reset_internal_e(const int *lo, const int *hi, real *up, const int *u_lo, const int *u_hi)
 {
     arr_t u = mk(up, u_lo, u_hi);
-    const double dual_energy_eta2 = 1.e-4;
+    const real dual_energy_eta2 = 1.e-4;
     LOOP3(lo, hi)
     {
-        double rhoInv = ONE / A(u, i, j, k, URHO);
-        double Up = A(u, i, j, k, UMX) * rhoInv;
-        double Vp = A(u, i, j, k, UMY) * rhoInv;
-        double Wp = A(u, i, j, k, UMZ) * rhoInv;
-        double ke = HALF * (Up * Up + Vp * Vp + Wp * Wp);
+        real rhoInv = ONE / A(u, i, j, k, URHO);
+        real Up = A(u, i, j, k, UMX) * rhoInv;
+        real Vp = A(u, i, j, k, UMY) * rhoInv;
+        real Wp = A(u, i, j, k, UMZ) * rhoInv;
+        real ke = HALF * (Up * Up + Vp * Vp + Wp * Wp);
         if (A(u, i, j, k, UEDEN) < ZERO) {
             if (A(u, i, j, k, UEINT) < ZERO) {
                 orc_eos_t st;
@@ -939,7 +939,7 @@ void orc_reset_internal_e(const int *lo, const int *hi, double *up, const int *u
             }
             A(u, i, j, k, UEDEN) = A(u, i, j, k, UEINT) + A(u, i, j, k, URHO) * ke;
         } else {
-            double rho_eint = A(u, i, j, k, UEDEN) - A(u, i, j, k, URHO) * ke;
+            real rho_eint = A(u, i, j, k, UEDEN) - A(u, i, j, k, URHO) * ke;
             if (rho_eint > ZERO && rho_eint / A(u, i, j, k, UEDEN) > dual_energy_eta2) {
                 A(u, i, j, k, UEINT) = rho_eint;
             } else if (A(u, i, j, k, UEINT) <= ZERO) {
@@ -948,7 +948,7 @@ void orc_reset_internal_e(const int *lo, const int *hi, double *up, const int *u
                 st.T = small_temp;
                 abar_zbar_rhoinv(&u, i, j, k, rhoInv, &st.abar, &st.zbar);
                 orc_eos(EOS_INPUT_RT, &st);
-                double eint_new = st.e;
+                real eint_new = st.e;
                 A(u, i, j, k, UEINT) = A(u, i, j, k, URHO) * eint_new;
             }
         }
@@ -956,12 +956,12 @@ void orc_reset_internal_e(const int *lo, const int *hi, double *up, const int *u
 }
 
 /* Source/Castro_nd.F90:253-297 */
-void orc_compute_temp(const int *lo, const int *hi, double *up, const int *u_lo, const int *u_hi)
+void orc_compute_temp(const int *lo, const int *hi, real *up, const int *u_lo, const int *u_hi)
 {
     arr_t u = mk(up, u_lo, u_hi);
     LOOP3(lo, hi)
     {
-        double rhoInv = ONE / A(u, i, j, k, URHO);
+        real rhoInv = ONE / A(u, i, j, k, URHO);
         orc_eos_t st;
         st.rho = A(u, i, j, k, URHO);
         st.T = A(u, i, j, k, UTEMP);
@@ -976,8 +976,8 @@ void orc_compute_temp(const int *lo, const int *hi, double *up, const int *u_lo,
 /* ------------------------------------------------------------------ */
 /* Source/Prob.F90:19-129, with the problem constants of Prob.F90:11-15
    passed in (the reference hard-codes them). */
-void orc_initdata(const int *lo, const int *hi, double *up, const int *u_lo, const int *u_hi,
-                  const double *dx, const double *problo, const double *probhi,
+void orc_initdata(const int *lo, const int *hi, real *up, const int *u_lo, const int *u_hi,
+                  const real *dx, const real *problo, const real *probhi,
                   const orc_prob_t *prob)
 {
     arr_t u = mk(up, u_lo, u_hi);
@@ -989,26 +989,26 @@ void orc_initdata(const int *lo, const int *hi, double *up, const int *u_lo, con
     st.abar = aion[0];
     st.zbar = zion[0];
     orc_eos(EOS_INPUT_RP, &st);
-    double e_ambient = st.e;
-    double r_init = prob->r_init;
-    double vctr = FOUR3RD * M_PI_ * (r_init * r_init * r_init);
-    double e_exp = prob->exp_energy / vctr / prob->dens_ambient;
-    double center[3];
+    real e_ambient = st.e;
+    real r_init = prob->r_init;
+    real vctr = FOUR3RD * M_PI_ * (r_init * r_init * r_init);
+    real e_exp = prob->exp_energy / vctr / prob->dens_ambient;
+    real center[3];
     for (int d = 0; d < 3; ++d) center[d] = (problo[d] + probhi[d]) / 2.e0;
 
     LOOP3(lo, hi)
     {
-        double xmin = problo[0] + dx[0] * (double)i;
-        double ymin = problo[1] + dx[1] * (double)j;
-        double zmin = problo[2] + dx[2] * (double)k;
+        real xmin = problo[0] + dx[0] * (real)i;
+        real ymin = problo[1] + dx[1] * (real)j;
+        real zmin = problo[2] + dx[2] * (real)k;
         int npert = 0, nambient = 0;
         for (int kk = 0; kk <= nsub - 1; ++kk) {
-            double zz = zmin + (dx[2] / (double)nsub) * (kk + 0.5e0);
+            real zz = zmin + (dx[2] / (real)nsub) * (kk + 0.5e0);
             for (int jj = 0; jj <= nsub - 1; ++jj) {
-                double yy = ymin + (dx[1] / (double)nsub) * (jj + 0.5e0);
+                real yy = ymin + (dx[1] / (real)nsub) * (jj + 0.5e0);
                 for (int ii = 0; ii <= nsub - 1; ++ii) {
-                    double xx = xmin + (dx[0] / (double)nsub) * (ii + 0.5e0);
-                    double dist = (center[0] - xx) * (center[0] - xx) +
+                    real xx = xmin + (dx[0] / (real)nsub) * (ii + 0.5e0);
+                    real dist = (center[0] - xx) * (center[0] - xx) +
                                   (center[1] - yy) * (center[1] - yy) +
                                   (center[2] - zz) * (center[2] - zz);
                     if (dist <= r_init * r_init) npert = npert + 1;
@@ -1016,7 +1016,7 @@ void orc_initdata(const int *lo, const int *hi, double *up, const int *u_lo, con
                 }
             }
         }
-        double eint = (npert * e_exp + nambient * e_ambient) / (double)(nsub * nsub * nsub);
+        real eint = (npert * e_exp + nambient * e_ambient) / (real)(nsub * nsub * nsub);
         A(u, i, j, k, URHO) = prob->dens_ambient;
         A(u, i, j, k, UMX) = 0.e0;
         A(u, i, j, k, UMY) = 0.e0;
@@ -1030,20 +1030,20 @@ void orc_initdata(const int *lo, const int *hi, double *up, const int *u_lo, con
 }
 
 /* Source/Castro_nd.F90:349-400 (note center = (probhi - problo)/2, :377) */
-void orc_calculate_blast_radius(const int *lo, const int *hi, const double *up, const int *u_lo,
-                                const int *u_hi, const double *dx, const double *problo,
-                                const double *probhi, double *blast_mass, double *blast_radius,
-                                double max_density)
+void orc_calculate_blast_radius(const int *lo, const int *hi, const real *up, const int *u_lo,
+                                const int *u_hi, const real *dx, const real *problo,
+                                const real *probhi, real *blast_mass, real *blast_radius,
+                                real max_density)
 {
     arr_t u = mk(up, u_lo, u_hi);
-    const double density_tolerance = 0.1;
-    double center[3];
+    const real density_tolerance = 0.1;
+    real center[3];
     for (int d = 0; d < 3; ++d) center[d] = (probhi[d] - problo[d]) / TWO;
     LOOP3(lo, hi)
     {
-        double x = problo[0] + (i + HALF) * dx[0] - center[0];
-        double y = problo[1] + (j + HALF) * dx[1] - center[1];
-        double z = problo[2] + (k + HALF) * dx[2] - center[2];
+        real x = problo[0] + (i + HALF) * dx[0] - center[0];
+        real y = problo[1] + (j + HALF) * dx[1] - center[1];
+        real z = problo[2] + (k + HALF) * dx[2] - center[2];
         if (fabs(A(u, i, j, k, URHO) - max_density) / max_density <= density_tolerance) {
             *blast_mass = *blast_mass + A(u, i, j, k, URHO);
             *blast_radius = *blast_radius + A(u, i, j, k, URHO) * sqrt(x * x + y * y + z * z);
@@ -1053,18 +1053,18 @@ void orc_calculate_blast_radius(const int *lo, const int *hi, const double *up, 
 
 /* Source/Castro_nd.F90:301-345 */
 void orc_denerror(const int *lo, const int *hi, signed char *tagp, const int *t_lo, const int *t_hi,
-                  const double *denp, const int *d_lo, const int *d_hi, signed char set,
+                  const real *denp, const int *d_lo, const int *d_hi, signed char set,
                   signed char clear)
 {
     (void)clear;
     arr_t den = mk(denp, d_lo, d_hi);
     long s1 = t_hi[0] - t_lo[0] + 1, s2 = s1 * (t_hi[1] - t_lo[1] + 1);
-    const double dengrad_rel = 0.25;
+    const real dengrad_rel = 0.25;
     LOOP3(lo, hi)
     {
-        double ax = fabs(A(den, i + 1, j, k, 0) - A(den, i, j, k, 0));
-        double ay = fabs(A(den, i, j + 1, k, 0) - A(den, i, j, k, 0));
-        double az = fabs(A(den, i, j, k + 1, 0) - A(den, i, j, k, 0));
+        real ax = fabs(A(den, i + 1, j, k, 0) - A(den, i, j, k, 0));
+        real ay = fabs(A(den, i, j + 1, k, 0) - A(den, i, j, k, 0));
+        real az = fabs(A(den, i, j, k + 1, 0) - A(den, i, j, k, 0));
         ax = fmax(ax, fabs(A(den, i, j, k, 0) - A(den, i - 1, j, k, 0)));
         ay = fmax(ay, fabs(A(den, i, j, k, 0) - A(den, i, j - 1, k, 0)));
         az = fmax(az, fabs(A(den, i, j, k, 0) - A(den, i, j, k - 1, 0)));

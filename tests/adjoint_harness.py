@@ -1,0 +1,116 @@
+"""Test harness around the discrete adjoint (mc_adjoint_*): a
+torch.autograd.Function for N CTU steps, the initial state as a
+differentiable function of the density field and the explosion energy, and
+the shell objective.  Needs a GPU."""
+import numpy as np
+import torch
+
+import minicastro as mc
+from helpers import smooth_direction  # noqa: F401
+
+NVAR, URHO, UMX, UMY, UMZ, UEDEN, UEINT, UTEMP, UFS = 20, 0, 1, 2, 3, 4, 5, 6, 7
+
+
+def to_torch(S):
+    """(nx, ny, nz, NVAR) Fortran-ordered numpy state -> (NVAR, nz, ny, nx) CUDA tensor."""
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(S).transpose(3, 2, 1, 0))).to("cuda")
+
+
+def init_state(cfg, exp_energy):
+    L = mc.Level(cfg["n_cell"], cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_PLAIN)
+    kw = dict(cfg)
+    kw["exp_energy"] = exp_energy
+    L.init_data(**kw)
+    S = L.state()
+    L.close()
+    return to_torch(S)
+
+
+class CTUSteps(torch.autograd.Function):
+    """U0 -> U_N: N CTU steps of the fused path with a frozen dt sequence.
+    backward() is the hand-written discrete adjoint; it applies to the most
+    recent forward pass of `adj`."""
+
+    @staticmethod
+    def forward(ctx, U0, adj, nsteps, dts):
+        U0 = U0.contiguous()
+        out = torch.empty_like(U0)
+        torch.cuda.synchronize()
+        adj.forward(U0.data_ptr(), out.data_ptr(), nsteps, dts)
+        ctx.adj = adj
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        g = gout.contiguous()
+        gin = torch.empty_like(g)
+        torch.cuda.synchronize()
+        ctx.adj.backward(g.data_ptr(), gin.data_ptr())
+        return gin, None, None, None
+
+
+class SedovAdjointProblem:
+    """Initial state U0(rho0, E) of initdata (Source/Prob.F90:19-129) with the
+    density replaced by a per-cell field: rho = rho0, rho E = rho e = rho0
+    eint(E), rho X_1 = rho0, T = 1e9, where eint(E) = a + b E per cell is
+    initdata's sub-sampled energy (affine in E)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        E = cfg["exp_energy"]
+        rho_amb = cfg["dens_ambient"]
+        S1 = init_state(cfg, E)
+        S0 = init_state(cfg, 0.0)
+        self.a = S0[UEINT] / rho_amb
+        self.b = (S1[UEINT] - S0[UEINT]) / rho_amb / E
+        self.T0 = S1[UTEMP].clone()
+        n = cfg["n_cell"]
+        rng = np.random.default_rng(cfg.get("seed", 11))
+        z = rng.standard_normal((n[2], n[1], n[0]))
+        self.rho0 = torch.from_numpy(rho_amb * (1.0 + cfg.get("noise", 0.01) * z)).to("cuda")
+        self.E = float(E)
+        lo, hi = np.asarray(cfg["prob_lo"]), np.asarray(cfg["prob_hi"])
+        dx = (hi - lo) / np.asarray(n)
+        ctr = 0.5 * (lo + hi)
+        xs = [lo[d] + dx[d] * (np.arange(n[d]) + 0.5) - ctr[d] for d in range(3)]
+        r = np.sqrt(xs[2][:, None, None] ** 2 + xs[1][None, :, None] ** 2 + xs[0][None, None, :] ** 2)
+        r0, r1 = cfg.get("shell", (0.2, 0.3))
+        self.mask = torch.from_numpy(((r >= r0) & (r <= r1)).astype(np.float64)).to("cuda")
+        self.adj = mc.Adjoint(n, cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["steps"])
+        self.nsteps = cfg["steps"]
+        self.dts = None
+
+    def U0(self, rho0, E):
+        eint = self.a + self.b * E
+        zero = torch.zeros_like(rho0)
+        comps = [zero] * NVAR
+        comps[URHO] = rho0
+        comps[UEDEN] = rho0 * eint
+        comps[UEINT] = rho0 * eint
+        comps[UTEMP] = self.T0
+        comps[UFS] = rho0
+        return torch.stack(comps)
+
+    def objective(self, UN):
+        return (self.mask * UN[URHO] ** 2).sum()
+
+    def J(self, rho0, E):
+        """Objective of a forward run with the frozen dt sequence (no graph)."""
+        with torch.no_grad():
+            U = self.U0(rho0, torch.tensor(E, dtype=torch.float64, device="cuda"))
+            out = CTUSteps.apply(U, self.adj, self.nsteps, self.dts)
+            return float(self.objective(out))
+
+    def gradient(self):
+        """(J, dJ/drho0, dJ/dE) by the adjoint; fixes the dt sequence on first use."""
+        if self.dts is None:
+            with torch.no_grad():
+                U = self.U0(self.rho0, torch.tensor(self.E, dtype=torch.float64, device="cuda"))
+                out = torch.empty_like(U)
+                self.dts = self.adj.forward(U.data_ptr(), out.data_ptr(), self.nsteps, None)
+        rho0 = self.rho0.clone().requires_grad_(True)
+        E = torch.tensor(self.E, dtype=torch.float64, device="cuda", requires_grad=True)
+        U = self.U0(rho0, E)
+        J = self.objective(CTUSteps.apply(U, self.adj, self.nsteps, self.dts))
+        J.backward()
+        return float(J.detach()), rho0.grad.detach(), float(E.grad)

@@ -52,6 +52,22 @@ def small_cfg(n=16, bc=1, r_init=0.12, **kw):
     return d
 
 
+def smooth_direction(rng, n, modes=6, kmax=3):
+    """Random smooth field on an n^3 grid (z, y, x order): a sum of `modes`
+    separable cosines with wavenumbers <= kmax, random phases and Gaussian
+    amplitudes."""
+    x = (np.arange(n) + 0.5) / n
+    v = np.zeros((n, n, n))
+    for _ in range(modes):
+        k = rng.integers(0, kmax + 1, 3)
+        ph = rng.uniform(0.0, 2.0 * np.pi, 3)
+        a = rng.standard_normal()
+        v += (a * np.cos(2 * np.pi * k[2] * x + ph[2])[:, None, None] *
+              np.cos(2 * np.pi * k[1] * x + ph[1])[None, :, None] *
+              np.cos(2 * np.pi * k[0] * x + ph[0])[None, None, :])
+    return v
+
+
 def rel_err(a, b):
     """max-norm relative difference per component (last axis)."""
     a = np.asarray(a).reshape(-1, a.shape[-1])

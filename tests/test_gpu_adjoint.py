@@ -1,0 +1,154 @@
+"""Discrete adjoint of the CTU step against central finite differences.
+
+The adjoint (mc_adjoint_*, wrapped as a torch.autograd.Function) gives the
+gradient of J = sum(rho^2) over the shell r in [0.2, 0.3] at the final step
+with respect to the initial density field rho0 = rho_amb (1 + 0.01 N(0,1))
+and to the explosion energy E.  The directional derivative along random
+directions (dRho, dE) must match central differences of the forward map run
+with the same frozen dt sequence.
+
+The discrete map is only piecewise smooth: PPM limiters, flattening clamps,
+Riemann sampling cases and floors switch branch along a perturbation.  A
+central difference over an interval that contains a switch is off by a
+fraction of the derivative jump, so the agreement reachable with finite
+differences depends on the direction: smooth density directions (random
+low-order cosine modes times rho0) cross few switches and are held to
+1e-6 at the better of eps = 1e-6 and 1e-7; per-cell white-noise directions
+cross hundreds of switches in the 1%-noise data and are only held to 1e-3.
+The exact check is against the CPU tangent-linear of the oracle (dual
+numbers through the restated reference arithmetic, same branches), which the
+adjoint must match to 1e-10 in every direction, white noise included.
+"""
+import numpy as np
+import pytest
+
+from minicastro import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_table():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import minicastro as mc
+    from helpers import table_bin
+    mc.load_table(table_bin())
+
+
+def directional(P, g_rho, g_E, v, vE, eps_list):
+    ad = float((g_rho * v).sum()) + g_E * vE
+    errs = []
+    for eps in eps_list:
+        jp = P.J(P.rho0 + eps * v, P.E + eps * vE)
+        jm = P.J(P.rho0 - eps * v, P.E - eps * vE)
+        fd = (jp - jm) / (2.0 * eps)
+        errs.append(abs(fd - ad) / abs(ad))
+    return ad, errs
+
+
+def run_checks(n, steps, nsmooth, nwhite, seed):
+    import torch
+    from adjoint_harness import SedovAdjointProblem, smooth_direction
+    P = SedovAdjointProblem(configs.adjoint_problem(n=n, steps=steps))
+    J, g_rho, g_E = P.gradient()
+    print("n=%d steps=%d J=%.12e dJ/dE=%.6e" % (n, steps, J, g_E), flush=True)
+    rng = np.random.default_rng(seed)
+    smooth, white = [], []
+    for q in range(nsmooth):
+        v = P.rho0 * torch.from_numpy(smooth_direction(rng, n)).to("cuda")
+        vE = P.E * rng.standard_normal()
+        ad, errs = directional(P, g_rho, g_E, v, vE, (1e-6, 1e-7))
+        print("smooth %2d: adjoint %.12e  rel err eps=1e-6 %.2e  eps=1e-7 %.2e" % (q, ad, errs[0], errs[1]),
+              flush=True)
+        smooth.append(min(errs))
+    for q in range(nwhite):
+        v = P.cfg["dens_ambient"] * torch.from_numpy(rng.standard_normal((n, n, n))).to("cuda")
+        vE = P.E * rng.standard_normal()
+        ad, errs = directional(P, g_rho, g_E, v, vE, (1e-6,))
+        print("white  %2d: adjoint %.12e  rel err eps=1e-6 %.2e" % (q, ad, errs[0]), flush=True)
+        white.append(errs[0])
+    _, errs = directional(P, g_rho, g_E, torch.zeros_like(P.rho0), P.E, (1e-6,))
+    print("E only    : rel err eps=1e-6 %.2e" % errs[0], flush=True)
+    return smooth, white, errs[0]
+
+
+def fd_verdict(smooth, white, e_only):
+    """Most smooth directions straddle no branch switch and must agree to
+    1e-6; a straddled switch (see the module docstring) may leave a few
+    directions off by the switch's derivative jump, bounded here by 1e-4."""
+    ok = sum(1 for e in smooth if e < 1e-6)
+    assert e_only < 1e-6
+    assert ok >= 0.75 * len(smooth), smooth
+    assert max(smooth) < 1e-4 and max(white) < 1e-3, (smooth, white)
+
+
+def test_adjoint_small_matches_fd():
+    fd_verdict(*run_checks(16, 3, 8, 2, seed=5))
+
+
+def test_adjoint_sedov64_10_steps_matches_fd():
+    fd_verdict(*run_checks(64, 10, 16, 2, seed=7))
+
+
+def test_adjoint_is_linear_in_the_seed():
+    """backward(a s1 + b s2) = a backward(s1) + b backward(s2)."""
+    import torch
+    import minicastro as mc
+    from adjoint_harness import SedovAdjointProblem
+    P = SedovAdjointProblem(configs.adjoint_problem(n=16, steps=2))
+    U = P.U0(P.rho0, torch.tensor(P.E, dtype=torch.float64, device="cuda")).contiguous()
+    out = torch.empty_like(U)
+    P.adj.forward(U.data_ptr(), out.data_ptr(), 2, None)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    s1 = torch.randn(U.shape, generator=g, dtype=torch.float64).to("cuda")
+    s2 = torch.randn(U.shape, generator=g, dtype=torch.float64).to("cuda")
+    r = []
+    for s in (s1, s2, 2.0 * s1 - 3.0 * s2):
+        o = torch.empty_like(U)
+        P.adj.backward(s.contiguous().data_ptr(), o.data_ptr())
+        r.append(o)
+    lhs, rhs = r[2], 2.0 * r[0] - 3.0 * r[1]
+    assert float((lhs - rhs).abs().max()) <= 1e-9 * float(rhs.abs().max())
+    # components outside the differentiated set are zero
+    assert float(r[0][6:].abs().max()) == 0.0
+
+
+def tangent_check(n, steps, ndir, seed, nthreads=16):
+    """GPU adjoint directional derivatives against the CPU tangent-linear
+    oracle (dual numbers through the restated reference arithmetic, same
+    branches): returns the relative differences."""
+    import os
+    import torch
+    from adjoint_harness import SedovAdjointProblem, smooth_direction
+    from tl_problem import OracleSedovProblem
+    cfg = configs.adjoint_problem(n=n, steps=steps)
+    P = SedovAdjointProblem(cfg)
+    J, g_rho, g_E = P.gradient()
+    O = OracleSedovProblem(cfg)
+    np.testing.assert_array_equal(O.rho0.transpose(2, 1, 0), P.rho0.cpu().numpy())
+    threads = max(1, min(nthreads, os.cpu_count() or 1))
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(ndir):
+        if q % 2 == 0:
+            v = P.rho0.cpu().numpy() * smooth_direction(rng, n)
+        else:
+            v = P.cfg["dens_ambient"] * rng.standard_normal((n, n, n))
+        vE = P.E * rng.standard_normal()
+        ad = float((g_rho * torch.from_numpy(v).to("cuda")).sum()) + g_E * vE
+        J_tl, tl, _ = O.tangent(np.ascontiguousarray(v.transpose(2, 1, 0)), vE, P.dts, nthreads=threads)
+        assert abs(J_tl - J) <= 1e-13 * abs(J), (J_tl, J)  # same forward map (sums differ in order)
+        out.append(abs(ad - tl) / abs(tl))
+        print("%s %2d: adjoint %.15e tangent-linear %.15e rel %.2e" %
+              ("smooth" if q % 2 == 0 else "white ", q, ad, tl, out[-1]), flush=True)
+    return out
+
+
+def test_adjoint_matches_cpu_tangent_linear_small():
+    assert max(tangent_check(16, 3, 4, seed=13)) < 1e-10
+
+
+def test_adjoint_matches_cpu_tangent_linear_sedov64():
+    assert max(tangent_check(64, 10, 4, seed=17)) < 1e-10

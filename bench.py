@@ -86,6 +86,45 @@ def cpu_baseline(cfg_full, n=64, steps=2):
                       "oracle OpenMP over (1024,16,16) tiles" % (n, steps)}
 
 
+def ensemble_bench(steps=50, n_inst=24):
+    """24 independent Sedov instances (sides in {32,48,64,96}, seeded
+    energies, radii, densities), every instance with its own dt, advanced
+    together with one launch per stage (hipGraph).  Aggregate zone-updates/s,
+    and every instance's dt sequence end point and state compared bit for bit
+    with the same instance advanced alone by the per-box path."""
+    import numpy as np
+    import minicastro as mc
+    from minicastro import configs
+    insts = configs.ensemble(n_inst, seed=7)
+    E = mc.Ensemble(insts)
+    E.init_data()
+    tuned = E.autotune(reps=2)
+    E.run(1, use_graph=True)      # graph capture
+    E.init_data()
+    E.run(1, use_graph=True)
+    E.init_data()
+    t0 = time.perf_counter()
+    E.run(steps, use_graph=True)
+    t1 = time.perf_counter()
+    zones = E.zones()
+    dt_equal = state_equal = 0
+    for m, d in enumerate(insts):
+        L = mc.Level(d["n_cell"], d["boxes"], d["prob_lo"], d["prob_hi"], d["bc"], mc.MODE_PLAIN)
+        L.init_data(**d)
+        L.run(steps)
+        dt, tm, st = E.clock(m)
+        dt_equal += int(dt == L.dt and tm == L.time and st == L.steps == steps)
+        state_equal += int(np.array_equal(E.state(m), L.state()))
+        L.close()
+    E.close()
+    return {"workload": "%d Sedov instances, sides %s" % (n_inst, sorted({d["n_cell"][0] for d in insts})),
+            "cells": zones, "steps": steps, "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "zone_updates_per_s": zones * steps / (t1 - t0),
+            "dt_bitwise_equal_sequential": "%d/%d" % (dt_equal, n_inst),
+            "state_bitwise_equal_sequential": "%d/%d" % (state_equal, n_inst),
+            "tiling": {k: list(v[0]) for k, v in tuned.items()}}
+
+
 def adjoint_bench(reps=3):
     """Discrete adjoint on the 64^3 Sedov adjoint workload, 10 steps: time of
     the checkpointing forward pass and of the adjoint pass (recompute + the
@@ -139,6 +178,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3, help="per-box comparison steps (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble run")
     ap.add_argument("--tiling-json", default=None,
                     help="take the stage tiling from config.tiling of an earlier bench.py JSON line "
                          "instead of running the autotuner")
@@ -224,6 +264,7 @@ def main():
         P.close()
 
     adjoint = None if a.no_adjoint else adjoint_bench()
+    ensemble = None if a.no_ensemble else ensemble_bench()
     cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
     out = {"metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -235,7 +276,8 @@ def main():
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,
                       "stage_ms_per_step": {k: v[0] / a.steps for k, v in stages.items() if v[1]}},
-           "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint}
+           "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
+           "ensemble": ensemble}
     print(json.dumps(out))
 
 

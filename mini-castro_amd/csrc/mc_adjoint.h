@@ -154,7 +154,9 @@ __device__ __forceinline__ EosD eos_dual(const EosTable &tab, D2 den, D2 temp, D
     if (xt.v < 0.0) xt = dc(0.0);
     const D2 mxt = 1.0 - xt;
 
-    DW6 D, dD, T, dT, ddT;
+    // density weights, then one temperature weight set at a time (keeps the
+    // live dual registers to two sets)
+    DW6 D, dD, T;
     D.w0 = dpsi0_(xd);
     D.w1 = dpsi1_(xd) * dd;
     D.w2 = dpsi2_(xd) * dd2;
@@ -173,24 +175,23 @@ __device__ __forceinline__ EosD eos_dual(const EosTable &tab, D2 den, D2 temp, D
     T.w0m = dpsi0_(mxt);
     T.w1m = -1.0 * dpsi1_(mxt) * dt;
     T.w2m = dpsi2_(mxt) * dt2;
-    dT.w0 = ddpsi0_(xt) * dti;
-    dT.w1 = ddpsi1_(xt);
-    dT.w2 = ddpsi2_(xt) * dt;
-    dT.w0m = -1.0 * ddpsi0_(mxt) * dti;
-    dT.w1m = ddpsi1_(mxt);
-    dT.w2m = -1.0 * ddpsi2_(mxt) * dt;
-    ddT.w0 = dddpsi0_(xt) * dt2i;
-    ddT.w1 = dddpsi1_(xt) * dti;
-    ddT.w2 = dddpsi2_(xt);
-    ddT.w0m = dddpsi0_(mxt) * dt2i;
-    ddT.w1m = -1.0 * dddpsi1_(mxt) * dti;
-    ddT.w2m = dddpsi2_(mxt);
-
     const D2 free = dh5(N, T, D);
-    const D2 df_t = dh5(N, dT, D);
-    const D2 df_tt = dh5(N, ddT, D);
     const D2 df_d = dh5(N, T, dD);
-    const D2 df_dt = dh5(N, dT, dD);
+    T.w0 = ddpsi0_(xt) * dti;
+    T.w1 = ddpsi1_(xt);
+    T.w2 = ddpsi2_(xt) * dt;
+    T.w0m = -1.0 * ddpsi0_(mxt) * dti;
+    T.w1m = ddpsi1_(mxt);
+    T.w2m = -1.0 * ddpsi2_(mxt) * dt;
+    const D2 df_t = dh5(N, T, D);
+    const D2 df_dt = dh5(N, T, dD);
+    T.w0 = dddpsi0_(xt) * dt2i;
+    T.w1 = dddpsi1_(xt) * dti;
+    T.w2 = dddpsi2_(xt);
+    T.w0m = dddpsi0_(mxt) * dt2i;
+    T.w1m = -1.0 * dddpsi1_(mxt) * dti;
+    T.w2m = dddpsi2_(mxt);
+    const D2 df_tt = dh5(N, T, D);
     const D2 xt0 = dxpsi0_(xt), xt1 = dxpsi1_(xt) * dt, xt0m = dxpsi0_(mxt), xt1m = -1.0 * dxpsi1_(mxt) * dt;
     const D2 x0 = dxpsi0_(xd), x1 = dxpsi1_(xd) * dd, x0m = dxpsi0_(mxd), x1m = -1.0 * dxpsi1_(mxd) * dd;
     D2 dpepdd = ye * dh3(N, xt0, xt1, xt0m, xt1m, x0, x1, x0m, x1m);

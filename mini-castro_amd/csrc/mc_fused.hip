@@ -247,9 +247,14 @@ __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
     face_riemann_h(x, I, d, i, j, k, L, R, o);
     store_tflux_h(x.FD[d], I, e, d, o);
     double *F = x.FD[d];
+    // a side whose weight is zero contributes +-0 to fpl*xl + fmr*xr: its
+    // species are not read
+    const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
 #pragma unroll 1
     for (int n = 0; n < NSPEC; ++n) {
-        const double gx = o.fpl * EA(x.SM[d], I, NFS, FS_X + n, e) + o.fmr * EA(x.SP[d], I, NFS, FS_X + n, e);
+        const double xl = useL ? EA(x.SM[d], I, NFS, FS_X + n, e) : 0.0;
+        const double xr = useR ? EA(x.SP[d], I, NFS, FS_X + n, e) : 0.0;
+        const double gx = o.fpl * xl + o.fmr * xr;
         EA(F, I, NFF, FF_X + n, e) = o.f_rho * gx;
     }
 }
@@ -294,12 +299,18 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
     face_riemann_h(x, I, t, i, j, k, side[0], side[1], o);
     double *F = x.FT[combo], *FDd = x.FD[d];
     store_tflux_h(F, I, e, t, o);
-#pragma unroll 1
+    // only the side(s) with a nonzero Riemann weight are corrected and read
+    const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
+#pragma unroll
     for (int n = 0; n < NSPEC; ++n) {
-        const double x0 = trans1_species(rho[0], EA(x.SM[t], I, NFS, FS_X + n, e), EA(FDd, I, NFF, FF_X + n, er[0]),
-                                         EA(FDd, I, NFF, FF_X + n, el[0]), cdt, rrnew[0]);
-        const double x1 = trans1_species(rho[1], EA(x.SP[t], I, NFS, FS_X + n, e), EA(FDd, I, NFF, FF_X + n, er[1]),
-                                         EA(FDd, I, NFF, FF_X + n, el[1]), cdt, rrnew[1]);
+        const double x0 = useL ? trans1_species(rho[0], EA(x.SM[t], I, NFS, FS_X + n, e),
+                                                EA(FDd, I, NFF, FF_X + n, er[0]), EA(FDd, I, NFF, FF_X + n, el[0]),
+                                                cdt, rrnew[0])
+                               : 0.0;
+        const double x1 = useR ? trans1_species(rho[1], EA(x.SP[t], I, NFS, FS_X + n, e),
+                                                EA(FDd, I, NFF, FF_X + n, er[1]), EA(FDd, I, NFF, FF_X + n, el[1]),
+                                                cdt, rrnew[1])
+                               : 0.0;
         EA(F, I, NFF, FF_X + n, e) = o.f_rho * (o.fpl * x0 + o.fmr * x1);
     }
 }
@@ -374,14 +385,19 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     EA(FN, I, NFN, UEINT, e) = AVN(UEINT, o.f_e);
     EA(FN, I, NFN, UTEMP, e) = 0.0;
     double fs[NSPEC], sum = 0.0;
+    const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
 #pragma unroll
     for (int n = 0; n < NSPEC; ++n) {
-        const double x0 = trans2_species(rho[0], EA(x.SM[d], I, NFS, FS_X + n, e), EA(F2, I, NFF, FF_X + n, r2[0]),
-                                         EA(F2, I, NFF, FF_X + n, l2[0]), EA(F3, I, NFF, FF_X + n, r3[0]),
-                                         EA(F3, I, NFF, FF_X + n, l2[0]), h2, h3, rrnew[0]);
-        const double x1 = trans2_species(rho[1], EA(x.SP[d], I, NFS, FS_X + n, e), EA(F2, I, NFF, FF_X + n, r2[1]),
-                                         EA(F2, I, NFF, FF_X + n, l2[1]), EA(F3, I, NFF, FF_X + n, r3[1]),
-                                         EA(F3, I, NFF, FF_X + n, l2[1]), h2, h3, rrnew[1]);
+        const double x0 = useL ? trans2_species(rho[0], EA(x.SM[d], I, NFS, FS_X + n, e),
+                                                EA(F2, I, NFF, FF_X + n, r2[0]), EA(F2, I, NFF, FF_X + n, l2[0]),
+                                                EA(F3, I, NFF, FF_X + n, r3[0]), EA(F3, I, NFF, FF_X + n, l2[0]),
+                                                h2, h3, rrnew[0])
+                               : 0.0;
+        const double x1 = useR ? trans2_species(rho[1], EA(x.SP[d], I, NFS, FS_X + n, e),
+                                                EA(F2, I, NFF, FF_X + n, r2[1]), EA(F2, I, NFF, FF_X + n, l2[1]),
+                                                EA(F3, I, NFF, FF_X + n, r3[1]), EA(F3, I, NFF, FF_X + n, l2[1]),
+                                                h2, h3, rrnew[1])
+                               : 0.0;
         fs[n] = AVN(UFS + n, o.f_rho * (o.fpl * x0 + o.fmr * x1));
         sum = sum + fs[n];
     }

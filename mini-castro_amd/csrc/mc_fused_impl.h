@@ -48,10 +48,21 @@ struct Ctx {
     EosTable tab;
 };
 
-__device__ __forceinline__ int find_inst(const Plan &p)
+// Logical brick of this workgroup.  Consecutive workgroups are dealt
+// round-robin to the 8 XCDs, each with its own L2; mapping workgroup b to
+// brick start(b mod 8) + b/8 gives every XCD one contiguous run of bricks, so
+// the neighbour cells and faces a stencil re-reads sit in the same L2.
+__device__ __forceinline__ int logical_block()
+{
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+__device__ __forceinline__ int find_inst(const Plan &p, int b)
 {
     int lo = 0, hi = p.ninst - 1;
-    const int b = blockIdx.x;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (p.blk0[mid] <= b) lo = mid;
@@ -63,11 +74,12 @@ __device__ __forceinline__ int find_inst(const Plan &p)
 // Cell of this thread in the plan's range; false if outside.
 __device__ __forceinline__ bool plan_cell(const Plan &p, int &m, int &i, int &j, int &k)
 {
-    m = find_inst(p);
+    const int lb = logical_block();
+    m = find_inst(p, lb);
     const FInst &I = p.inst[m];
     const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.n[2] + p.add;
     const int nb0 = (e0 + p.bx - 1) / p.bx, nb1 = (e1 + p.by - 1) / p.by;
-    const int b = blockIdx.x - p.blk0[m];
+    const int b = lb - p.blk0[m];
     const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
     const int t = threadIdx.x;
     const int ti = t % p.bx, tj = (t / p.bx) % p.by, tk = t / (p.bx * p.by);

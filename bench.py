@@ -23,6 +23,7 @@ for p in (os.path.join(ROOT, "mini-castro_amd"), os.path.join(ROOT, "oracle"), o
 
 METRIC = "zones/s at 256^3 Sedov FP64, batched vs per-box (1e-12 rel); adjoint/fwd time"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+FP64_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
 
 # Algorithmic HBM bytes per unit of each fused stage: every input element
 # read once and every output element written once (doubles).
@@ -60,6 +61,20 @@ def measured_traffic(kernel, workload):
         for ln in f:
             w = ln.split()
             if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel and w[2] == workload:
+                return float(w[3])
+    return None
+
+
+def measured_fp64_ops(kernel, workload):
+    """FP64 operations issued per launch of `kernel` on `workload` from the
+    committed rocprofv3 SQ_INSTS_VALU_*_F64 summary, or None."""
+    path = os.path.join(ROOT, "profiles", "fp64_fused_stages_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            w = ln.split()
+            if len(w) == 4 and w[0] == "fp64_ops_per_launch" and w[1] == kernel and w[2] == workload:
                 return float(w[3])
     return None
 
@@ -179,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble run")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="only time the adjoint workload and print its JSON (for profiling)")
     ap.add_argument("--tiling-json", default=None,
                     help="take the stage tiling from config.tiling of an earlier bench.py JSON line "
                          "instead of running the autotuner")
@@ -201,6 +218,10 @@ def main():
     import minicastro as mc
     from minicastro import configs
 
+    if a.only_adjoint:
+        mc.load_table()
+        print(json.dumps(adjoint_bench()))
+        return
     cfg = configs.config(a.config)
     n = cfg["n_cell"]
     zones = int(n[0]) * int(n[1]) * int(n[2])
@@ -248,6 +269,16 @@ def main():
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "mc::K_" + dom,
                 "avg_launch_ms": tot_ms / nl, "algorithmic_bytes_per_launch": traffic_alg,
                 "traffic_GB_per_s": (traffic / avg_s / 1e9) if traffic else None}
+
+    # FP64 pipe load of the EOS-bound stages (clean_state + ctoprim, update + estdt)
+    fp64 = {}
+    for st in ("prim", "update"):
+        ops = measured_fp64_ops("mc::K_" + st, cfg["name"])
+        if ops and stages[st][1]:
+            rate = ops / (stages[st][0] / stages[st][1] / 1e3)
+            fp64[st] = {"bound": "fp64 valu", "achieved": rate / 1e12, "peak": FP64_PEAK / 1e12,
+                        "unit": "TFLOP/s", "frac": rate / FP64_PEAK, "ops_per_launch": ops}
+    roofline["fp64_stages"] = fp64 or None
 
     # per-box path (the reference's kernel order) on the same workload
     plain = None

@@ -303,6 +303,7 @@ def main():
            "config": {"workload": cfg["name"], "n_cell": list(n), "boxes": int(len(cfg["boxes"])),
                       "path": "batched fused, one launch per stage over all boxes",
                       "graph_replay_ms_per_step": graph_ms, "steps_advanced": fused_steps_done,
+                      "fom_zones_per_usec": value / 1e6,
                       "tiling": tiling, "tiling_source": "autotuner" if tuned else a.tiling_json,
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,

@@ -111,7 +111,7 @@ def lib():
     if _lib is None:
         p = lib_path()
         if not os.path.exists(p):
-            raise RuntimeError("libmccastro.so is not built (run __graft_entry__.build() or make -C %s)" % CSRC)
+            raise RuntimeError("libmccastro.so is not built (run make -C %s)" % CSRC)
         L = C.CDLL(p)
         for name, res, args in SIGS:
             f = getattr(L, name)

@@ -74,7 +74,7 @@ def run_checks(n, steps, nsmooth, nwhite, seed):
     return smooth, white, errs[0]
 
 
-def fd_verdict(smooth, white, e_only):
+def check_fd_results(smooth, white, e_only):
     """Most smooth directions straddle no branch switch and must agree to
     1e-6; a straddled switch (see the module docstring) may leave a few
     directions off by the switch's derivative jump, bounded here by 1e-4."""
@@ -85,11 +85,11 @@ def fd_verdict(smooth, white, e_only):
 
 
 def test_adjoint_small_matches_fd():
-    fd_verdict(*run_checks(16, 3, 8, 2, seed=5))
+    check_fd_results(*run_checks(16, 3, 8, 2, seed=5))
 
 
 def test_adjoint_sedov64_10_steps_matches_fd():
-    fd_verdict(*run_checks(64, 10, 16, 2, seed=7))
+    check_fd_results(*run_checks(64, 10, 16, 2, seed=7))
 
 
 def test_adjoint_is_linear_in_the_seed():

@@ -51,17 +51,22 @@ struct ACtx {
     double *SMb[3], *SPb[3], *FDb[3], *FTb[6], *FNb[3];
 };
 
-__device__ __forceinline__ void aadd(double *p, double v)
+// device-scope FP64 atomic add (the hardware global_atomic_add_f64)
+__device__ __forceinline__ void aadd(gdouble *p, double v)
 {
-    if (v != 0.0) unsafeAtomicAdd(p, v);
+    if (v != 0.0) {
+        [[clang::atomic(ignore_denormal_mode)]] {
+            (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
-__device__ __forceinline__ double *EAp(double *a, const FInst &I, int C, int c, long idx)
+__device__ __forceinline__ gdouble *EAp(double *a, const FInst &I, int C, int c, idx_t idx)
 {
-    return a + (long)C * I.offE + (long)c * I.nE + idx;
+    return &at(a + ((long)C * I.offE + (long)c * I.nE), idx);
 }
-__device__ __forceinline__ double *CAp(double *a, const FInst &I, int C, int c, long idx)
+__device__ __forceinline__ gdouble *CAp(double *a, const FInst &I, int C, int c, idx_t idx)
 {
-    return a + (long)C * I.offD + (long)c * I.nD + idx;
+    return &at(a + ((long)C * I.offD + (long)c * I.nD), idx);
 }
 
 // conserved-state adjoint component h -> flux component in the FN array
@@ -73,12 +78,12 @@ __global__ __launch_bounds__(256) void A_update(Plan p, Ctx x, ACtx a)
     int m, i, j, k;
     if (!plan_cell(p, m, i, j, k)) return;
     const FInst &I = p.inst[m];
-    const long c = cidx(I, i, j, k);
+    const idx_t c = cidx(I, i, j, k);
     const double dx0 = I.dx[0], dx1 = I.dx[1], dx2 = I.dx[2];
     const double a0 = dx1 * dx2, a1 = dx0 * dx2, a2 = dx0 * dx1;
     const double volinv = 1.0 / (dx0 * dx1 * dx2);
-    const long e = eidx(I, i, j, k);
-    const long ex = eidx(I, i + 1, j, k), ey = eidx(I, i, j + 1, k), ez = eidx(I, i, j, k + 1);
+    const idx_t e = eidx(I, i, j, k);
+    const idx_t ex = eidx(I, i + 1, j, k), ey = eidx(I, i, j + 1, k), ez = eidx(I, i, j, k + 1);
     double *F0 = x.FN[0], *F1 = x.FN[1], *F2 = x.FN[2];
     const double dt = x.dt_cur[m];
     double u0[NVAR], ua[NVAR], u1[NVAR], u2[NVAR];
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(256) void A_update(Plan p, Ctx x, ACtx a)
     for (int h = 0; h < NHB; ++h) CA(a.Ubo, I, NHB, h, c) = ub[h];
 
     const double ad[3] = {a0, a1, a2};
-    const long ep[3] = {ex, ey, ez};
+    const idx_t ep[3] = {ex, ey, ez};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double *FB = a.FNb[d];
@@ -150,8 +155,8 @@ __device__ __forceinline__ void face_riemann_hb(const Ctx &x, const ACtx &a, con
                                                 int k, const FaceH &L, const FaceH &R, const RiemannHB &rb,
                                                 FaceHB &Lb, FaceHB &Rb)
 {
-    const long cr = cidx(I, i, j, k);
-    const long cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
+    const idx_t cr = cidx(I, i, j, k);
+    const idx_t cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
     double cLb = 0.0, cRb = 0.0, glb = 0.0, grb = 0.0;
     riemann_hydro_b(L, R, CA(x.Q, I, NFQ, FQ_C, cl), CA(x.Q, I, NFQ, FQ_C, cr), CA(x.Q, I, NFQ, FQ_G, cl),
                     CA(x.Q, I, NFQ, FQ_G, cr), rb, Lb, Rb, cLb, cRb, glb, grb);
@@ -179,7 +184,7 @@ __device__ __forceinline__ QStateHB from_faceb(const FaceHB &f, int d)
     return o;
 }
 
-__device__ __forceinline__ void add_stateb(double *S, const FInst &I, long e, const QInHB &b)
+__device__ __forceinline__ void add_stateb(double *S, const FInst &I, idx_t e, const QInHB &b)
 {
     aadd(EAp(S, I, NSB, SB_RHO, e), b.rho);
     aadd(EAp(S, I, NSB, SB_U, e), b.u);
@@ -189,7 +194,7 @@ __device__ __forceinline__ void add_stateb(double *S, const FInst &I, long e, co
     aadd(EAp(S, I, NSB, SB_RE, e), b.rhoe);
 }
 
-__device__ __forceinline__ void add_faceb(double *S, const FInst &I, long e, int d, const FaceHB &f)
+__device__ __forceinline__ void add_faceb(double *S, const FInst &I, idx_t e, int d, const FaceHB &f)
 {
     const int un = SB_U + d, ut = d == 0 ? SB_V : SB_U, utt = d == 2 ? SB_V : SB_W;
     aadd(EAp(S, I, NSB, SB_RHO, e), f.rho);
@@ -201,7 +206,7 @@ __device__ __forceinline__ void add_faceb(double *S, const FInst &I, long e, int
 }
 
 // adjoint of a transverse-stage flux entry (URHO..UEDEN + Godunov p, u)
-__device__ __forceinline__ void add_tfluxb(double *FB, const FInst &I, long e, const double *vb, double gpb,
+__device__ __forceinline__ void add_tfluxb(double *FB, const FInst &I, idx_t e, const double *vb, double gpb,
                                            double gub)
 {
 #pragma unroll
@@ -211,7 +216,7 @@ __device__ __forceinline__ void add_tfluxb(double *FB, const FInst &I, long e, c
 }
 
 // Riemann output adjoints stored in a transverse-flux adjoint entry
-__device__ __forceinline__ RiemannHB load_tfluxb(double *FB, const FInst &I, long e, int d)
+__device__ __forceinline__ RiemannHB load_tfluxb(double *FB, const FInst &I, idx_t e, int d)
 {
     const int mn = FB_MX + d, mt = d == 0 ? FB_MY : FB_MX, mtt = d == 2 ? FB_MY : FB_MZ;
     RiemannHB r;
@@ -242,11 +247,11 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
     const int ed[3] = {d == 0, d == 1, d == 2}, e1[3] = {t1 == 0, t1 == 1, t1 == 2}, e2[3] = {t2 == 0, t2 == 1, t2 == 2};
     const double dt = x.dt_cur[m];
     const double h2 = 0.5 * dt / I.dx[t1], h3 = 0.5 * dt / I.dx[t2];
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     const int p2 = pair_index(t1, t2), p3 = pair_index(t2, t1);
     double *F2 = x.FT[p2], *F3 = x.FT[p3];
     FaceH side[2];
-    long l2[2], r2[2], r3[2];
+    idx_t l2[2], r2[2], r3[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const int sh = s - 1;
@@ -268,7 +273,7 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
         side[s] = to_face(o, d);
     }
     // artificial viscosity (forward: Source/Castro_hydro_nd.F90:174-245)
-    long nodes[4];
+    idx_t nodes[4];
     if (d == 0) {
         nodes[0] = eidx(I, i, j, k);
         nodes[1] = eidx(I, i, j + 1, k);
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
     const double draw = FOURTH * (EA(x.DV, I, 1, 0, nodes[0]) + EA(x.DV, I, 1, 0, nodes[1]) +
                                   EA(x.DV, I, 1, 0, nodes[2]) + EA(x.DV, I, 1, 0, nodes[3]));
     const double div1 = 0.1 * fmin(0.0, draw);
-    const long cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
+    const idx_t cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
     const double dxd = I.dx[d];
     double fnb[NNB];
 #pragma unroll
@@ -334,7 +339,7 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
         load_tflux_h(F2, I, l2[s], f2l, pg2m, ug2m);
         load_tflux_h(F3, I, r3[s], f3r, pg3p, ug3p);
         load_tflux_h(F3, I, l2[s], f3l, pg3m, ug3m);
-        const long cg = cidx(I, il, jl, kl);
+        const idx_t cg = cidx(I, il, jl, kl);
         const double gamc = CA(x.Q, I, NFQ, FQ_G, cg);
         const QStateHB ob = from_faceb(sb[s], d);
         QInHB lqb = {0, 0, 0, 0, 0, 0};
@@ -361,12 +366,12 @@ __global__ __launch_bounds__(256) void A_trans1(Plan p, Ctx x, ACtx a)
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
     if (c[t] < 0 || c[d] < 0 || c[d] > I.n[d] - 1) return;
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     const int et[3] = {t == 0, t == 1, t == 2}, ed[3] = {d == 0, d == 1, d == 2};
     const double cdtdx = x.dt_cur[m] / I.dx[d];
     const double cdt = cdtdx / 3.0;
     FaceH side[2];
-    long el[2], er[2];
+    idx_t el[2], er[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const int sh = s - 1;
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(256) void A_trans1(Plan p, Ctx x, ACtx a)
         double fr[UEDEN + 1], fl[UEDEN + 1], pgp, ugp, pgm, ugm;
         load_tflux_h(x.FD[d], I, er[s], fr, pgp, ugp);
         load_tflux_h(x.FD[d], I, el[s], fl, pgm, ugm);
-        const long cg = cidx(I, il, jl, kl);
+        const idx_t cg = cidx(I, il, jl, kl);
         const double gamc = CA(x.Q, I, NFQ, FQ_G, cg);
         const QStateHB ob = from_faceb(sb[s], t);
         QInHB lqb = {0, 0, 0, 0, 0, 0};
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(256) void A_flux1(Plan p, Ctx x, ACtx a)
     const FInst &I = p.inst[m];
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     if (pos < 0) return;
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     FaceH L, R;
     load_fs_h(x.SM[d], I, e, d, L);
     load_fs_h(x.SP[d], I, e, d, R);
@@ -439,13 +444,13 @@ __global__ __launch_bounds__(256) void A_trace(Plan p, Ctx x, ACtx a)
     const bool do_p = pos >= 0;
     const bool do_m = pos <= I.n[d] - 1;
     const int cun = FQ_U + d, cut = d == 0 ? FQ_V : (d == 1 ? FQ_W : FQ_U), cutt = d == 0 ? FQ_W : (d == 1 ? FQ_U : FQ_V);
-    const long c0 = cidx(I, i, j, k);
+    const idx_t c0 = cidx(I, i, j, k);
     const double cc = CA(x.Q, I, NFQ, FQ_C, c0);
     const double un = CA(x.Q, I, NFQ, cun, c0);
-    const long ec = eidx(I, i, j, k);
+    const idx_t ec = eidx(I, i, j, k);
     const double flatn = EA(x.FL, I, 1, 0, ec);
     const double dtdx = x.dt_cur[m] / I.dx[d];
-    long cs[5];
+    idx_t cs[5];
 #pragma unroll
     for (int s = -2; s <= 2; ++s) cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
     const int vars[7] = {FQ_RHO, cun, FQ_P, FQ_RE, cut, cutt, FQ_G};
@@ -468,7 +473,7 @@ __global__ __launch_bounds__(256) void A_trace(Plan p, Ctx x, ACtx a)
     ppm_int_wave(SMv[6], SPv[6], S6[6], un - cc, dtdx, dummy, gam_m);
     ppm_int_wave(SMv[6], SPv[6], S6[6], un + cc, dtdx, gam_p, dummy);
 
-    const long ep = ec, em = eidx(I, i + di, j + dj, k + dk);
+    const idx_t ep = ec, em = eidx(I, i + di, j + dj, k + dk);
     const int sun = SB_U + d, sut = d == 0 ? SB_V : (d == 1 ? SB_W : SB_U), sutt = d == 0 ? SB_W : (d == 1 ? SB_U : SB_V);
     double Ipb[3][4] = {{0}}, Imb[3][4] = {{0}};
     double gam_mb = 0.0, gam_pb = 0.0, Ip_utb = 0.0, Im_utb = 0.0, Ip_uttb = 0.0, Im_uttb = 0.0;
@@ -534,7 +539,7 @@ __global__ __launch_bounds__(256) void A_flat(Plan p, Ctx x, ACtx a)
     int m, i, j, k;
     if (!plan_cell(p, m, i, j, k)) return;
     const FInst &I = p.inst[m];
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     const double fb = EA(a.FLb, I, 1, 0, e);
     if (fb != 0.0) {
         double cur = 1.0;
@@ -544,7 +549,7 @@ __global__ __launch_bounds__(256) void A_flat(Plan p, Ctx x, ACtx a)
             double p7[7], u7[7];
 #pragma unroll
             for (int s = -3; s <= 3; ++s) {
-                const long c = cidx(I, i + s * ei, j + s * ej, k + s * ek);
+                const idx_t c = cidx(I, i + s * ei, j + s * ej, k + s * ek);
                 p7[s + 3] = CA(x.Q, I, NFQ, FQ_P, c);
                 u7[s + 3] = CA(x.Q, I, NFQ, FQ_U + d, c);
             }
@@ -557,12 +562,12 @@ __global__ __launch_bounds__(256) void A_flat(Plan p, Ctx x, ACtx a)
         if (arg >= 0) {
             const int ei = (arg == 0), ej = (arg == 1), ek = (arg == 2);
             double p7[7], u7[7], p7b[7] = {0, 0, 0, 0, 0, 0, 0};
-            long cc[7];
+            idx_t cc[7];
 #pragma unroll
             for (int s = -3; s <= 3; ++s) {
                 cc[s + 3] = cidx(I, i + s * ei, j + s * ej, k + s * ek);
                 p7[s + 3] = CA(x.Q, I, NFQ, FQ_P, cc[s + 3]);
-                u7[s + 3] = CA(x.Q, I, NFQ, FQ_U + arg, cc[s + 3]);
+                u7[s + 3] = CA_lane(x.Q, I, NFQ, FQ_U + arg, cc[s + 3]);  // arg differs per lane
             }
             flatten_dir_b(p7, u7, fb, p7b);
 #pragma unroll
@@ -609,7 +614,7 @@ __global__ __launch_bounds__(256) void A_prim(Plan p, Ctx x, ACtx a)
     int m, i, j, k;
     if (!plan_cell(p, m, i, j, k)) return;
     const FInst &I = p.inst[m];
-    const long c = cidx(I, i, j, k);
+    const idx_t c = cidx(I, i, j, k);
     double u0[NVAR], ua[NVAR], u1[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) u0[n] = a.Uold[(long)NVAR * I.offD + (long)n * I.nD + c];

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void K_est(Plan p, Ctx x)
     const bool ok = plan_cell(p, m, i, j, k);
     const FInst &I = p.inst[m];
     if (ok) {
-        const long c = cidx(I, i, j, k);
+        const idx_t c = cidx(I, i, j, k);
         double u[NVAR];
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void K_prim(Plan p, Ctx x)
     int m, i, j, k;
     if (!plan_cell(p, m, i, j, k)) return;
     const FInst &I = p.inst[m];
-    const long c = cidx(I, i, j, k);
+    const idx_t c = cidx(I, i, j, k);
     double u[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
@@ -106,14 +106,14 @@ __global__ __launch_bounds__(256) void K_flat(Plan p, Ctx x)
     int m, i, j, k;
     if (!plan_cell(p, m, i, j, k)) return;
     const FInst &I = p.inst[m];
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     double flatn = 1.0;
     for (int d = 0; d < 3; ++d) {
         const int ei = (d == 0), ej = (d == 1), ek = (d == 2);
         double p7[7], u7[7];
 #pragma unroll
         for (int s = -3; s <= 3; ++s) {
-            const long c = cidx(I, i + s * ei, j + s * ej, k + s * ek);
+            const idx_t c = cidx(I, i + s * ei, j + s * ej, k + s * ek);
             p7[s + 3] = CA(x.Q, I, NFQ, FQ_P, c);
             u7[s + 3] = CA(x.Q, I, NFQ, FQ_U + d, c);
         }
@@ -155,12 +155,12 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     const bool do_p = pos >= 0;
     const bool do_m = pos <= I.n[d] - 1;
     const int cun = FQ_U + d, cut = d == 0 ? FQ_V : (d == 1 ? FQ_W : FQ_U), cutt = d == 0 ? FQ_W : (d == 1 ? FQ_U : FQ_V);
-    const long c0 = cidx(I, i, j, k);
+    const idx_t c0 = cidx(I, i, j, k);
     const double cc = CA(x.Q, I, NFQ, FQ_C, c0);
     const double un = CA(x.Q, I, NFQ, cun, c0);
     const double flatn = EA(x.FL, I, 1, 0, eidx(I, i, j, k));
     const double dtdx = x.dt_cur[m] / I.dx[d];
-    long cs[5];
+    idx_t cs[5];
 #pragma unroll
     for (int s = -2; s <= 2; ++s) cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
 
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
         (void)a;
         (void)b;
     }
-    const long ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
+    const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
     double *SP = x.SP[d], *SM = x.SM[d];
     for (int ns = 0; ns < NSPEC; ++ns) {
         double s[5], sm, sp, ipv, imv;
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
     const FInst &I = p.inst[m];
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     if (pos < 0) return;
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     FaceH L, R;
     load_fs_h(x.SM[d], I, e, d, L);
     load_fs_h(x.SP[d], I, e, d, R);
@@ -271,12 +271,12 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
     const int c[3] = {i, j, k};
     // range: t-face 0..n_t, d-cells 0..n_d-1, the third direction -1..n
     if (c[t] < 0 || c[d] < 0 || c[d] > I.n[d] - 1) return;
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     const int et[3] = {t == 0, t == 1, t == 2}, ed[3] = {d == 0, d == 1, d == 2};
     const double cdtdx = x.dt_cur[m] / I.dx[d];
     const double cdt = cdtdx / 3.0;
     FaceH side[2];
-    long el[2], er[2];
+    idx_t el[2], er[2];
     double rho[2], rrnew[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -333,10 +333,10 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     const int ed[3] = {d == 0, d == 1, d == 2}, e1[3] = {t1 == 0, t1 == 1, t1 == 2}, e2[3] = {t2 == 0, t2 == 1, t2 == 2};
     const double dt = x.dt_cur[m];
     const double h2 = 0.5 * dt / I.dx[t1], h3 = 0.5 * dt / I.dx[t2];
-    const long e = eidx(I, i, j, k);
+    const idx_t e = eidx(I, i, j, k);
     double *F2 = x.FT[pair_index(t1, t2)], *F3 = x.FT[pair_index(t2, t1)];
     FaceH side[2];
-    long l2[2], r2[2], r3[2];
+    idx_t l2[2], r2[2], r3[2];
     double rho[2], rrnew[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     else div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j + 1, k) + DVN(i + 1, j + 1, k));
 #undef DVN
     div1 = 0.1 * fmin(0.0, div1);
-    const long cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
+    const idx_t cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
     const double dxd = I.dx[d];
 #define AVN(n, fval) ((fval) + dxd * (div1 * (CA(x.U1, I, NVAR, n, cr) - CA(x.U1, I, NVAR, n, cl))))
     const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
@@ -419,12 +419,12 @@ __global__ __launch_bounds__(256) void K_update(Plan p, Ctx x)
     const bool ok = plan_cell(p, m, i, j, k);
     const FInst &I = p.inst[m];
     if (ok) {
-        const long c = cidx(I, i, j, k);
+        const idx_t c = cidx(I, i, j, k);
         const double dx0 = I.dx[0], dx1 = I.dx[1], dx2 = I.dx[2];
         const double a0 = dx1 * dx2, a1 = dx0 * dx2, a2 = dx0 * dx1;
         const double volinv = 1.0 / (dx0 * dx1 * dx2);
-        const long e = eidx(I, i, j, k);
-        const long ex = eidx(I, i + 1, j, k), ey = eidx(I, i, j + 1, k), ez = eidx(I, i, j, k + 1);
+        const idx_t e = eidx(I, i, j, k);
+        const idx_t ex = eidx(I, i + 1, j, k), ey = eidx(I, i, j + 1, k), ez = eidx(I, i, j, k + 1);
         double *F0 = x.FN[0], *F1 = x.FN[1], *F2 = x.FN[2];
         const double dt = x.dt_cur[m];
         double u[NVAR];
@@ -600,6 +600,11 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
         I.periodic = sp.bc == 0;
         I.nD = (long)I.n[0] * I.n[1] * I.n[2];
         I.nE = (long)I.ext[0] * I.ext[1] * I.ext[2];
+        if (I.nE >= (1L << 29)) {  // per-instance byte offsets are 32-bit
+            set_error("fused engine: an instance may hold at most 2^29 extended cells");
+            delete F;
+            return nullptr;
+        }
         I.offD = offD;
         I.offE = offE;
         offD += I.nD;

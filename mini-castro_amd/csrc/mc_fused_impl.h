@@ -95,29 +95,65 @@ __device__ __forceinline__ int bmap(int i, int n, int periodic)
     return periodic ? ((i % n) + n) % n : min(max(i, 0), n - 1);
 }
 
-// index of a (boundary-mapped) cell in the cell space of instance I
-__device__ __forceinline__ long cidx(const FInst &I, int i, int j, int k)
+// Cell and face indices within one instance are 32-bit and so are the byte
+// offsets of an instance's component arrays (fewer than 2^29 extended cells
+// per instance, checked at creation).  The array base of a component is
+// uniform across a workgroup (one instance per workgroup), so every access is
+// a scalar base plus a 32-bit per-lane byte offset.
+typedef unsigned idx_t;
+
+// Every array base CA / EA index is uniform across a wavefront: a workgroup
+// belongs to one instance, and its direction / pair index comes from
+// blockIdx.y or compile-time loops (a component chosen per lane goes through
+// CA_lane instead).  Stating it keeps the base in scalar registers (one 32-bit
+// vector offset per access instead of a 64-bit address).
+__device__ __forceinline__ double *uniform_ptr(double *p)
 {
-    return (long)bmap(i, I.n[0], I.periodic) +
-           (long)I.n[0] * (bmap(j, I.n[1], I.periodic) + (long)I.n[1] * bmap(k, I.n[2], I.periodic));
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<double *>(((unsigned long long)hi << 32) | lo);
+}
+
+// a double in the global address space (loads and stores through it are
+// global_load / global_store with the scalar base)
+typedef __attribute__((address_space(1))) double gdouble;
+
+__device__ __forceinline__ gdouble &at(double *base, idx_t idx)
+{
+    typedef __attribute__((address_space(1))) char gchar;
+    gchar *g = (gchar *)uniform_ptr(base);
+    return *(gdouble *)(g + idx * 8u);
+}
+
+// index of a (boundary-mapped) cell in the cell space of instance I
+__device__ __forceinline__ idx_t cidx(const FInst &I, int i, int j, int k)
+{
+    return (idx_t)bmap(i, I.n[0], I.periodic) +
+           (idx_t)I.n[0] * ((idx_t)bmap(j, I.n[1], I.periodic) + (idx_t)I.n[1] * (idx_t)bmap(k, I.n[2], I.periodic));
 }
 // index of an extended cell / face (i,j,k in [-1, n])
-__device__ __forceinline__ long eidx(const FInst &I, int i, int j, int k)
+__device__ __forceinline__ idx_t eidx(const FInst &I, int i, int j, int k)
 {
-    return (long)(i + 1) + (long)I.ext[0] * ((j + 1) + (long)I.ext[1] * (k + 1));
+    return (idx_t)(i + 1) + (idx_t)I.ext[0] * ((idx_t)(j + 1) + (idx_t)I.ext[1] * (idx_t)(k + 1));
 }
 // component c of a cell array with C components
-__device__ __forceinline__ double &CA(double *a, const FInst &I, int C, int c, long idx)
+__device__ __forceinline__ gdouble &CA(double *a, const FInst &I, int C, int c, idx_t idx)
+{
+    return at(uniform_ptr(a + ((long)C * I.offD + (long)c * I.nD)), idx);
+}
+// component c chosen per lane (c not uniform across the wavefront)
+__device__ __forceinline__ double CA_lane(const double *a, const FInst &I, int C, int c, idx_t idx)
 {
     return a[(long)C * I.offD + (long)c * I.nD + idx];
 }
-__device__ __forceinline__ double &EA(double *a, const FInst &I, int C, int c, long idx)
+__device__ __forceinline__ gdouble &EA(double *a, const FInst &I, int C, int c, idx_t idx)
 {
-    return a[(long)C * I.offE + (long)c * I.nE + idx];
+    return at(uniform_ptr(a + ((long)C * I.offE + (long)c * I.nE)), idx);
 }
 
 // Hydro part (rho, velocities permuted to direction d, p, rhoe) of a state
-__device__ __forceinline__ void load_fs_h(double *S, const FInst &I, long e, int d, FaceH &f)
+__device__ __forceinline__ void load_fs_h(double *S, const FInst &I, idx_t e, int d, FaceH &f)
 {
     const int un = FS_U + d, ut = d == 0 ? FS_V : FS_U, utt = d == 2 ? FS_V : FS_W;
     f.rho = EA(S, I, NFS, FS_RHO, e);
@@ -128,7 +164,7 @@ __device__ __forceinline__ void load_fs_h(double *S, const FInst &I, long e, int
     f.rhoe = EA(S, I, NFS, FS_RE, e);
 }
 
-__device__ __forceinline__ void load_qin_h(double *S, const FInst &I, long e, QInH &q)
+__device__ __forceinline__ void load_qin_h(double *S, const FInst &I, idx_t e, QInH &q)
 {
     q.rho = EA(S, I, NFS, FS_RHO, e);
     q.u = EA(S, I, NFS, FS_U, e);
@@ -152,7 +188,7 @@ __device__ __forceinline__ FaceH to_face(const QStateH &o, int d)
 }
 
 // Non-species part of a transverse-stage flux; momenta in absolute order.
-__device__ __forceinline__ void store_tflux_h(double *F, const FInst &I, long e, int d, const RiemannH &o)
+__device__ __forceinline__ void store_tflux_h(double *F, const FInst &I, idx_t e, int d, const RiemannH &o)
 {
     const int mn = FF_MX + d, mt = d == 0 ? FF_MY : FF_MX, mtt = d == 2 ? FF_MY : FF_MZ;
     EA(F, I, NFF, FF_RHO, e) = o.f_rho;
@@ -168,14 +204,14 @@ __device__ __forceinline__ void store_tflux_h(double *F, const FInst &I, long e,
 __device__ __forceinline__ void face_riemann_h(const Ctx &x, const FInst &I, int d, int i, int j, int k,
                                                const FaceH &L, const FaceH &R, RiemannH &o)
 {
-    const long cr = cidx(I, i, j, k);
-    const long cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
+    const idx_t cr = cidx(I, i, j, k);
+    const idx_t cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
     riemann_hydro(L, R, CA(x.Q, I, NFQ, FQ_C, cl), CA(x.Q, I, NFQ, FQ_C, cr), CA(x.Q, I, NFQ, FQ_G, cl),
                   CA(x.Q, I, NFQ, FQ_G, cr), o);
 }
 
 // URHO..UEDEN of a transverse-stage flux and its Godunov p, u
-__device__ __forceinline__ void load_tflux_h(double *F, const FInst &I, long e, double *v, double &gp, double &gu)
+__device__ __forceinline__ void load_tflux_h(double *F, const FInst &I, idx_t e, double *v, double &gp, double &gu)
 {
 #pragma unroll
     for (int n = 0; n <= UEDEN; ++n) v[n] = EA(F, I, NFF, FF_RHO + n, e);

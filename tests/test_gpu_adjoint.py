@@ -115,7 +115,7 @@ def test_adjoint_is_linear_in_the_seed():
     assert float(r[0][6:].abs().max()) == 0.0
 
 
-def tangent_check(n, steps, ndir, seed, nthreads=16):
+def tangent_check(n, steps, ndir, seed, nthreads=16, **over):
     """GPU adjoint directional derivatives against the CPU tangent-linear
     oracle (dual numbers through the restated reference arithmetic, same
     branches): returns the relative differences."""
@@ -124,6 +124,7 @@ def tangent_check(n, steps, ndir, seed, nthreads=16):
     from adjoint_harness import SedovAdjointProblem, smooth_direction
     from tl_problem import OracleSedovProblem
     cfg = configs.adjoint_problem(n=n, steps=steps)
+    cfg.update(over)
     P = SedovAdjointProblem(cfg)
     J, g_rho, g_E = P.gradient()
     O = OracleSedovProblem(cfg)
@@ -152,3 +153,8 @@ def test_adjoint_matches_cpu_tangent_linear_small():
 
 def test_adjoint_matches_cpu_tangent_linear_sedov64():
     assert max(tangent_check(64, 10, 4, seed=17)) < 1e-10
+
+
+def test_adjoint_periodic_matches_cpu_tangent_linear():
+    """Periodic boundaries: the reverse kernels scatter through the wrap map."""
+    assert max(tangent_check(16, 4, 4, seed=23, bc=0, r_init=0.3, shell=(0.25, 0.5))) < 1e-10

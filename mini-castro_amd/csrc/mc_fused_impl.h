@@ -48,18 +48,6 @@ struct Ctx {
     EosTable tab;
 };
 
-// Logical brick of this workgroup.  Consecutive workgroups are dealt
-// round-robin to the 8 XCDs, each with its own L2; mapping workgroup b to
-// brick start(b mod 8) + b/8 gives every XCD one contiguous run of bricks, so
-// the neighbour cells and faces a stencil re-reads sit in the same L2.
-__device__ __forceinline__ int logical_block()
-{
-    const int b = blockIdx.x, nb = gridDim.x;
-    const int q = nb >> 3, r = nb & 7;
-    const int x = b & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
-
 __device__ __forceinline__ int find_inst(const Plan &p, int b)
 {
     int lo = 0, hi = p.ninst - 1;

@@ -104,7 +104,7 @@ static int eos_upload(const double *tab)
 #define MC_CELL(it)                                                      \
     int i, j, k;                                                         \
     {                                                                    \
-        long t_ = (long)blockIdx.x * blockDim.x + threadIdx.x;           \
+        long t_ = (long)logical_block() * blockDim.x + threadIdx.x;      \
         if (!iter_cell(it, t_, i, j, k)) return;                         \
     }
 

@@ -455,23 +455,29 @@ __global__ __launch_bounds__(256) void A_trace(Plan p, Ctx x, ACtx a)
     for (int s = -2; s <= 2; ++s) cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
     const int vars[7] = {FQ_RHO, cun, FQ_P, FQ_RE, cut, cutt, FQ_G};
     const int qbv[7] = {QB_RHO, QB_U + d, QB_P, QB_RE, QB_U + (cut - FQ_U), QB_U + (cutt - FQ_U), QB_G};
-    double S[7][5], SMv[7], SPv[7], S6[7];
-#pragma unroll
-    for (int vv = 0; vv < 7; ++vv) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) S[vv][q] = CA(x.Q, I, NFQ, vars[vv], cs[q]);
-        ppm_reconstruct(S[vv], flatn, SMv[vv], SPv[vv]);
-        S6[vv] = ppm_s6(SMv[vv], SPv[vv], S[vv][2]);
-    }
+    // Forward: the PPM integrals the projection reads.  Edge values are not
+    // kept; the reverse sweep reads each stencil again (L2 hits) and rebuilds
+    // them, which keeps the kernel within one register file per wave.
     const double spd[3] = {un - cc, un, un + cc};
     double Ip[3][4], Im[3][4];
+    double gam_m, gam_p;
 #pragma unroll
-    for (int vv = 0; vv < 4; ++vv)
+    for (int vv = 0; vv < 7; ++vv) {
+        if (vv == 4 || vv == 5) continue;
+        double s5[5], sm, sp;
 #pragma unroll
-        for (int w = 0; w < 3; ++w) ppm_int_wave(SMv[vv], SPv[vv], S6[vv], spd[w], dtdx, Ip[w][vv], Im[w][vv]);
-    double gam_m, gam_p, dummy;
-    ppm_int_wave(SMv[6], SPv[6], S6[6], un - cc, dtdx, dummy, gam_m);
-    ppm_int_wave(SMv[6], SPv[6], S6[6], un + cc, dtdx, gam_p, dummy);
+        for (int q = 0; q < 5; ++q) s5[q] = CA(x.Q, I, NFQ, vars[vv], cs[q]);
+        ppm_reconstruct(s5, flatn, sm, sp);
+        const double s6 = ppm_s6(sm, sp, s5[2]);
+        if (vv < 4) {
+#pragma unroll
+            for (int w = 0; w < 3; ++w) ppm_int_wave(sm, sp, s6, spd[w], dtdx, Ip[w][vv], Im[w][vv]);
+        } else {
+            double dummy;
+            ppm_int_wave(sm, sp, s6, un - cc, dtdx, dummy, gam_m);
+            ppm_int_wave(sm, sp, s6, un + cc, dtdx, gam_p, dummy);
+        }
+    }
 
     const idx_t ep = ec, em = eidx(I, i + di, j + dj, k + dk);
     const int sun = SB_U + d, sut = d == 0 ? SB_V : (d == 1 ? SB_W : SB_U), sutt = d == 0 ? SB_W : (d == 1 ? SB_U : SB_V);
@@ -494,28 +500,33 @@ __global__ __launch_bounds__(256) void A_trace(Plan p, Ctx x, ACtx a)
     double unb = 0.0, ccb = 0.0, flatb = 0.0;
 #pragma unroll
     for (int vv = 0; vv < 7; ++vv) {
+        double s5[5], SMv, SPv;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s5[q] = CA(x.Q, I, NFQ, vars[vv], cs[q]);
+        ppm_reconstruct(s5, flatn, SMv, SPv);
+        const double S6 = ppm_s6(SMv, SPv, s5[2]);
         double smb = 0.0, spb = 0.0, s6b = 0.0;
         if (vv < 4) {
 #pragma unroll
             for (int w = 0; w < 3; ++w) {
                 double spdb = 0.0;
-                ppm_int_wave_b(SMv[vv], SPv[vv], S6[vv], spd[w], dtdx, Ipb[w][vv], Imb[w][vv], smb, spb, s6b, spdb);
+                ppm_int_wave_b(SMv, SPv, S6, spd[w], dtdx, Ipb[w][vv], Imb[w][vv], smb, spb, s6b, spdb);
                 unb += spdb;
                 if (w == 0) ccb -= spdb;
                 if (w == 2) ccb += spdb;
             }
         } else if (vv < 6) {
             double spdb = 0.0;
-            ppm_int_wave_b(SMv[vv], SPv[vv], S6[vv], un, dtdx, vv == 4 ? Ip_utb : Ip_uttb, vv == 4 ? Im_utb : Im_uttb,
+            ppm_int_wave_b(SMv, SPv, S6, un, dtdx, vv == 4 ? Ip_utb : Ip_uttb, vv == 4 ? Im_utb : Im_uttb,
                            smb, spb, s6b, spdb);
             unb += spdb;
         } else {
             double spdb = 0.0;
-            ppm_int_wave_b(SMv[6], SPv[6], S6[6], un - cc, dtdx, 0.0, gam_mb, smb, spb, s6b, spdb);
+            ppm_int_wave_b(SMv, SPv, S6, un - cc, dtdx, 0.0, gam_mb, smb, spb, s6b, spdb);
             unb += spdb;
             ccb -= spdb;
             spdb = 0.0;
-            ppm_int_wave_b(SMv[6], SPv[6], S6[6], un + cc, dtdx, gam_pb, 0.0, smb, spb, s6b, spdb);
+            ppm_int_wave_b(SMv, SPv, S6, un + cc, dtdx, gam_pb, 0.0, smb, spb, s6b, spdb);
             unb += spdb;
             ccb += spdb;
         }
@@ -524,7 +535,7 @@ __global__ __launch_bounds__(256) void A_trace(Plan p, Ctx x, ACtx a)
         s5b[2] += 6.0 * s6b;
         smb -= 3.0 * s6b;
         spb -= 3.0 * s6b;
-        ppm_reconstruct_b(S[vv], flatn, smb, spb, s5b, flatb);
+        ppm_reconstruct_b(s5, flatn, smb, spb, s5b, flatb);
 #pragma unroll
         for (int q = 0; q < 5; ++q) aadd(CAp(a.Qb, I, NQB, qbv[vv], cs[q]), s5b[q]);
     }

@@ -65,6 +65,26 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def reference_flop_per_zone_update(workload):
+    """FP64 operations per zone-update of the reference algorithm, counted by
+    the op-count build of the CPU restatement (tests/opcount_zone_update.py),
+    from the committed summary, or None."""
+    path = os.path.join(ROOT, "profiles", "fp64_work_per_zone_update_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            w = ln.split()
+            if len(w) == 3 and w[0] == "flop_per_zone_update" and w[1] == workload:
+                return float(w[2])
+    return None
+
+
+# Unit bytes of a zone-update: the 20 conserved components read and written
+# once (Source/Castro_advance.cpp:11-100 reads S_old and writes S_new).
+B_ALG_PER_ZONE_UPDATE = 2 * 20 * 8
+
+
 def measured_fp64_ops(kernel, workload):
     """FP64 operations issued per launch of `kernel` on `workload` from the
     committed rocprofv3 SQ_INSTS_VALU_*_F64 summary, or None."""
@@ -79,26 +99,64 @@ def measured_fp64_ops(kernel, workload):
     return None
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cfg_full, n=64, steps=2):
     """The oracle (CPU restatement, OpenMP over (1024,16,16) tiles) on an n^3
-    single-box sample of the same problem; zone-updates/s."""
+    single-box sample of the same problem, in zone-updates/s: the timing build
+    (-O3, AVX2/FMA) on up to 16 host threads and, for one more step from the
+    same state, on one thread; and the parity build (no FMA contraction, the
+    one the tests compare against) on the same threads."""
     import numpy as np
     import orc
-    from helpers import oracle_level
+    from helpers import table_array
     from minicastro import configs
     threads = max(1, min(16, os.cpu_count() or 1))
     cfg = dict(cfg_full)
     cfg.update(n_cell=(n, n, n), boxes=configs.grid_boxes(n, n))
-    L = oracle_level(cfg, nthreads=threads)
-    L.coarse_step()  # first step (the initial-temperature Newton solves are longer); untimed
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        L.coarse_step()
-    t1 = time.perf_counter()
-    del np, orc
-    return {"value": n ** 3 * steps / (t1 - t0), "unit": "zone-updates/s", "cores": threads, "kind": "port",
+    tab = np.ascontiguousarray(table_array(), dtype=np.float64)
+
+    def level(lib, nthreads):
+        lib.orc_eos_init_arrays(orc.ptr(tab))
+        L = orc.Level(cfg["n_cell"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["boxes"], nthreads,
+                      _library=lib)
+        L.init_data(cfg["p_ambient"], cfg["dens_ambient"], cfg["exp_energy"], cfg["r_init"], cfg["nsub"])
+        return L
+
+    def timed(L, k):
+        t0 = time.perf_counter()
+        for _ in range(k):
+            L.coarse_step()
+        return n ** 3 * k / (time.perf_counter() - t0)
+
+    rate = {}
+    for name, lib in (("fast", orc.fast_lib()), ("parity", orc.lib())):
+        L = level(lib, threads)
+        L.coarse_step()  # first step (the initial-temperature Newton solves are longer); untimed
+        rate[name] = timed(L, steps)
+        if name == "fast":
+            one = orc.Level(cfg["n_cell"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["boxes"], 1,
+                            _library=lib)
+            one.set_state(L.state())
+            rate["fast_1core"] = timed(one, 1)
+            del one
+        del L
+    return {"value": rate["fast"], "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "value_1core": rate["fast_1core"], "value_parity_build": rate["parity"],
+            "build": orc.FAST_FLAGS, "parity_build": "-O2 -ffp-contract=off",
+            "host": {"cpu": _cpu_model(), "os_cpu_count": os.cpu_count()},
             "sample": "%d^3 single-box Sedov (same problem constants), %d steps after one untimed step, "
-                      "oracle OpenMP over (1024,16,16) tiles" % (n, steps)}
+                      "oracle OpenMP over (1024,16,16) tiles; value_1core: one more step on one thread"
+                      % (n, steps)}
 
 
 def ensemble_bench(steps=50, n_inst=24):
@@ -279,6 +337,25 @@ def main():
             fp64[st] = {"bound": "fp64 valu", "achieved": rate / 1e12, "peak": FP64_PEAK / 1e12,
                         "unit": "TFLOP/s", "frac": rate / FP64_PEAK, "ops_per_launch": ops}
     roofline["fp64_stages"] = fp64 or None
+
+    # whole step: work of the reference algorithm per zone-update at the
+    # measured rate, and the FP64 operations the fused stages issue
+    # (rocprof counters, summed over the stages of one step)
+    w_flop = reference_flop_per_zone_update(cfg["name"])
+    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in STAGE_BYTES]
+    hw_step = sum(hw) if all(hw) else None
+    roofline["step"] = {
+        "unit_work": "zone-update",
+        "reference_flop_per_zone_update": w_flop,
+        "fp64_achieved_TFLOPs": (value * w_flop / 1e12) if w_flop else None,
+        "fp64_frac": (value * w_flop / FP64_PEAK) if w_flop else None,
+        "issued_fp64_ops_per_zone_update": (hw_step / zones) if hw_step else None,
+        "issued_fp64_TFLOPs": (hw_step / (ms_step / 1e3) / 1e12) if hw_step else None,
+        "issued_fp64_frac": (hw_step / (ms_step / 1e3) / FP64_PEAK) if hw_step else None,
+        "bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE,
+        "hbm_achieved_GBs": value * B_ALG_PER_ZONE_UPDATE / 1e9,
+        "hbm_frac": value * B_ALG_PER_ZONE_UPDATE / HBM_PEAK,
+    }
 
     # per-box path (the reference's kernel order) on the same workload
     plain = None

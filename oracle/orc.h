@@ -21,10 +21,14 @@
 #define ORC_H
 
 /* Scalar type of the restated arithmetic: double; the tangent-linear build
-   (compiled as C++ with ORC_TL) substitutes a forward-mode dual number. */
+   (compiled as C++ with ORC_TL) substitutes a forward-mode dual number, the
+   op-count build (C++ with ORC_CNT) a double that counts its operations. */
 #ifdef ORC_TL
 #include "orc_dual.h"
 typedef orc_dual real;
+#elif defined(ORC_CNT)
+#include "orc_count.h"
+typedef orc_cnt real;
 #else
 typedef double real;
 #endif

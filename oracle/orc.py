@@ -41,12 +41,11 @@ def build():
 _lib = None
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = C.CDLL(LIB_PATH)
+def _bind(path):
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    if True:
         L.orc_level_create.restype = C.c_void_p
         L.orc_level_state.restype = _dp
         L.orc_level_source.restype = _dp
@@ -68,7 +67,13 @@ def lib():
         L.orc_eos_init.argtypes = [C.c_char_p]
         L.orc_ppm_reconstruct.argtypes = [_dp, C.c_double, _dp, _dp]
         L.orc_ppm_int_profile.argtypes = [C.c_double] * 6 + [_dp, _dp]
-        _lib = L
+    return L
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _bind(LIB_PATH)
     return _lib
 
 
@@ -158,8 +163,8 @@ def call(name, *args):
 
 # ---------------------------------------------------------------- level
 class Level:
-    def __init__(self, n_cell, prob_lo, prob_hi, bc, boxes, nthreads=1):
-        self.L = lib()
+    def __init__(self, n_cell, prob_lo, prob_hi, bc, boxes, nthreads=1, _library=None):
+        self.L = _library or lib()
         boxes = np.asarray(boxes, dtype=np.int32).reshape(-1, 6)
         self.boxes = boxes
         bx = np.ascontiguousarray(boxes.ravel())
@@ -312,3 +317,56 @@ class TLLevel:
     def step(self, dt):
         self.L.orc_level_advance(self.h, DualT(float(dt), 0.0))
         self.L.orc_level_post_timestep(self.h)
+
+
+# ------------------------------------------------------------------ op count
+# build/liboracle_cnt.so: the same restatement, single-threaded, with a scalar
+# that counts its floating-point operations (values equal the double build).
+CNT_PATH = os.path.join(HERE, "build", "liboracle_cnt.so")
+OPCOUNT_FIELDS = ("add", "mul", "div", "sqrt", "log")
+_cnt = None
+
+
+def cnt_lib():
+    global _cnt
+    if _cnt is None:
+        _cnt = _bind(CNT_PATH)
+        _cnt.orc_opcount_read.argtypes = [C.POINTER(C.c_ulonglong)]
+    return _cnt
+
+
+class CntLevel(Level):
+    """Oracle level of the op-count build; opcount() reads the operations
+    executed since the last opcount_reset()."""
+
+    def __init__(self, n_cell, prob_lo, prob_hi, bc, boxes):
+        super().__init__(n_cell, prob_lo, prob_hi, bc, boxes, nthreads=1, _library=cnt_lib())
+
+    def opcount_reset(self):
+        self.L.orc_opcount_reset()
+
+    def opcount(self):
+        out = (C.c_ulonglong * 5)()
+        self.L.orc_opcount_read(out)
+        return dict(zip(OPCOUNT_FIELDS, (int(v) for v in out)))
+
+
+def cnt_eos_init_arrays(tab):
+    t = np.ascontiguousarray(tab, dtype=np.float64)
+    cnt_lib().orc_eos_init_arrays(t.ctypes.data_as(_dp))
+
+
+# ------------------------------------------------------------------ timing
+# build/liboracle_fast.so: the same restatement built for speed (-O3, AVX2,
+# FMA contraction allowed); its results may differ in the last bits, so only
+# the CPU-baseline timing uses it.
+FAST_PATH = os.path.join(HERE, "build", "liboracle_fast.so")
+FAST_FLAGS = "-O3 -march=x86-64-v3 -ffp-contract=fast"
+_fast = None
+
+
+def fast_lib():
+    global _fast
+    if _fast is None:
+        _fast = _bind(FAST_PATH)
+    return _fast

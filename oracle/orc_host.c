@@ -404,3 +404,17 @@ real orc_level_coarse_step(orc_level_t *L, real stop_time)
     L->steps += 1;
     return L->dt;
 }
+
+#ifdef ORC_CNT
+/* op-count build: the operation counters of orc_count.h */
+void orc_opcount_read(unsigned long long *out5)
+{
+    out5[0] = orc_opc.add;
+    out5[1] = orc_opc.mul;
+    out5[2] = orc_opc.div;
+    out5[3] = orc_opc.sqrt;
+    out5[4] = orc_opc.log;
+}
+
+void orc_opcount_reset(void) { orc_opc = orc_opcount_t{0, 0, 0, 0, 0}; }
+#endif

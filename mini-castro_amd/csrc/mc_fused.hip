@@ -412,7 +412,10 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 // fill_hydro_source (Source/Castro_hydro_nd.F90:344-441), Saxpy and the two
 // clean_state calls of Castro_advance.cpp:74-78 / Castro.cpp:369, then the
 // next step's estdt (Source/timestep.F90:12-72) min-reduced per instance.
-__global__ __launch_bounds__(256) void K_update(Plan p, Ctx x)
+// Held to 256 registers (2 waves per SIMD): the few spilled values cost less
+// than the second wave gains on the EOS table gathers (6.1 -> 5.5 ms at 256^3;
+// the same limit slows K_prim, which spills more).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K_update(Plan p, Ctx x)
 {
     int m, i, j, k;
     double v = DBL_MAX;

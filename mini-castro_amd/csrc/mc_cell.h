@@ -116,38 +116,60 @@ struct Prim {
     double gamc, cs, dpdr_e, dpde;
 };
 
-__device__ __forceinline__ void ctoprim_cell(const double *u, const EosTable &tab, Prim &q)
+// ctoprim before the EOS call: density, velocities, internal energy (the
+// dual-energy choice), mass fractions and their abar / zbar.
+struct PrimPre {
+    double rho, u, v, w, e, abar, zbar, T0;
+};
+
+__device__ __forceinline__ void ctoprim_pre(const double *u, PrimPre &a, double *xs)
 {
     const double rho = u[URHO];
-    q.rho = rho;
+    a.rho = rho;
     const double rhoinv = 1.0 / rho;
-    q.u = u[UMX] * rhoinv;
-    q.v = u[UMY] * rhoinv;
-    q.w = u[UMZ] * rhoinv;
-    const double kineng = HALF * rho * (q.u * q.u + q.v * q.v + q.w * q.w);
+    a.u = u[UMX] * rhoinv;
+    a.v = u[UMY] * rhoinv;
+    a.w = u[UMZ] * rhoinv;
+    const double kineng = HALF * rho * (a.u * a.u + a.v * a.v + a.w * a.w);
     const double E = u[UEDEN];
-    double e;
-    if ((E - kineng) / E > dual_energy_eta1) e = (E - kineng) * rhoinv;
-    else e = u[UEINT] * rhoinv;
+    if ((E - kineng) / E > dual_energy_eta1) a.e = (E - kineng) * rhoinv;
+    else a.e = u[UEINT] * rhoinv;
     double s1 = 0.0, s2 = 0.0;
 #pragma unroll
     for (int n = 0; n < NSPEC; ++n) {
-        q.x[n] = u[UFS + n] * rhoinv;
-        s1 = s1 + q.x[n] * ainv(n);
+        xs[n] = u[UFS + n] * rhoinv;
+        s1 = s1 + xs[n] * ainv(n);
     }
 #pragma unroll
-    for (int n = 0; n < NSPEC; ++n) s2 = s2 + q.x[n] * zion(n) * ainv(n);
-    const double abar = 1.0 / s1;
-    const double zbar = abar * s2;
-    const EosRE r = eos_re<true>(tab, rho, u[UTEMP], e, abar, zbar);
+    for (int n = 0; n < NSPEC; ++n) s2 = s2 + xs[n] * zion(n) * ainv(n);
+    a.abar = 1.0 / s1;
+    a.zbar = a.abar * s2;
+    a.T0 = u[UTEMP];
+}
+
+// ctoprim's EOS call and the fields it gives
+__device__ __forceinline__ void ctoprim_eos(const EosTable &tab, const PrimPre &a, Prim &q)
+{
+    const EosRE r = eos_re<true>(tab, a.rho, a.T0, a.e, a.abar, a.zbar);
     q.T = r.T;
-    q.rhoe = e * rho;
+    q.rhoe = a.e * a.rho;
     q.p = r.p;
     q.game = r.p / q.rhoe + 1.0;
     q.gamc = r.gam1;
     q.cs = r.cs;
     q.dpdr_e = r.dpdr_e;
     q.dpde = r.dpde;
+}
+
+__device__ __forceinline__ void ctoprim_cell(const double *u, const EosTable &tab, Prim &q)
+{
+    PrimPre a;
+    ctoprim_pre(u, a, q.x);
+    q.rho = a.rho;
+    q.u = a.u;
+    q.v = a.v;
+    q.w = a.w;
+    ctoprim_eos(tab, a, q);
 }
 
 // CFL time step of one cell (Source/timestep.F90:44-64)

@@ -67,8 +67,9 @@ __global__ __launch_bounds__(256) void K_est(Plan p, Ctx x)
     if (threadIdx.x == 0) atomic_min_d(&x.dt_est[m], v);
 }
 
-// S_old clean (Castro_advance.cpp:51), Sborder clean (:66), ctoprim
-__global__ __launch_bounds__(256) void K_prim(Plan p, Ctx x)
+// S_old clean (Castro_advance.cpp:51), Sborder clean (:66), ctoprim.
+// Held to 2 waves per SIMD like K_update (3.85 -> 3.65 ms at 256^3).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K_prim(Plan p, Ctx x)
 {
     int m, i, j, k;
     if (!plan_cell(p, m, i, j, k)) return;
@@ -81,16 +82,23 @@ __global__ __launch_bounds__(256) void K_prim(Plan p, Ctx x)
     clean_cell(u, x.tab);
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) CA(x.U1, I, NVAR, n, c) = u[n];
+    // the EOS-free primitives are stored before the EOS call, so they are
+    // not held in registers through it
+    PrimPre a;
+    {
+        double xs[NSPEC];
+        ctoprim_pre(u, a, xs);
+        CA(x.Q, I, NFQ, FQ_RHO, c) = a.rho;
+        CA(x.Q, I, NFQ, FQ_U, c) = a.u;
+        CA(x.Q, I, NFQ, FQ_V, c) = a.v;
+        CA(x.Q, I, NFQ, FQ_W, c) = a.w;
+#pragma unroll
+        for (int n = 0; n < NSPEC; ++n) CA(x.Q, I, NFQ, FQ_X + n, c) = xs[n];
+    }
     Prim q;
-    ctoprim_cell(u, x.tab, q);
-    CA(x.Q, I, NFQ, FQ_RHO, c) = q.rho;
-    CA(x.Q, I, NFQ, FQ_U, c) = q.u;
-    CA(x.Q, I, NFQ, FQ_V, c) = q.v;
-    CA(x.Q, I, NFQ, FQ_W, c) = q.w;
+    ctoprim_eos(x.tab, a, q);
     CA(x.Q, I, NFQ, FQ_P, c) = q.p;
     CA(x.Q, I, NFQ, FQ_RE, c) = q.rhoe;
-#pragma unroll
-    for (int n = 0; n < NSPEC; ++n) CA(x.Q, I, NFQ, FQ_X + n, c) = q.x[n];
     CA(x.Q, I, NFQ, FQ_C, c) = q.cs;
     CA(x.Q, I, NFQ, FQ_G, c) = q.gamc;
 }

@@ -246,6 +246,34 @@ int mc_level_run_stage(mc_level_t *L, int stage);
    tuned); either may be NULL. */
 int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms);
 
+/* ---- batched advance over caller-owned FABs ----
+   Castro::advance followed by post_timestep's clean_state
+   (Source/Castro_advance.cpp:11-100, Source/Castro.cpp:343-369) for every box
+   of a single level at once, replacing the per-box construct_hydro_source
+   loop (Source/Castro_hydro.cpp:31-397) with one launch per stage over all
+   boxes.  The caller keeps its MultiFabs: each work item names a valid box
+   and the device FABs holding it, S_old (read) and S_new (written), each as
+   (pointer, FAB lo, FAB hi) in Fortran order with NVAR components, i
+   fastest; a FAB may be larger than its box (ghost cells are neither read
+   nor written) and S_new may alias S_old.  The boxes must tile [0, n_cell)
+   exactly once.  Ghost data come from the boxes themselves (periodic images
+   or first-order extrapolation at the domain boundary), as FillPatch
+   provides them (Source/Castro_advance.cpp:62).  S_new equals what the
+   per-box path (and mc_level in either mode) computes, bit for bit.
+   dt_est (may be NULL) receives Castro::estTimeStep of S_new, cfl * min
+   dx/(c+|u|) (Source/Castro.cpp:175-216).  Runs on the stream given to
+   mc_set_stream and returns after it has drained.  The engine built for the
+   last geometry and box list is kept for the next call (calls from several
+   threads are serialised); mc_advance_batch_release frees it. */
+typedef struct {
+    int lo[3], hi[3];                               /* valid box */
+    const double *s_old; int old_lo[3], old_hi[3];  /* S_old FAB */
+    double *s_new; int new_lo[3], new_hi[3];        /* S_new FAB */
+} mc_batch_box_t;
+int mc_advance_batch(const int *n_cell, const double *prob_lo, const double *prob_hi, int bc, int nbox,
+                     const mc_batch_box_t *work, double dt, double *dt_est);
+void mc_advance_batch_release(void);
+
 /* ---- ensembles of independent single-box instances ----
    Every instance is its own domain [0, n_cell) with its own problem constants
    and time-step control; the fused engine advances all of them with one

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -112,6 +113,26 @@ __global__ void k_fill(long n, double *p, double v)
     if (t < n) p[t] = v;
 }
 
+// the boxes lie in [0, n) and cover it exactly once
+static bool tiles_domain(const std::vector<Box> &boxes, const int *n)
+{
+    long v = 0;
+    for (const Box &b : boxes) {
+        for (int d = 0; d < 3; ++d)
+            if (b.lo[d] < 0 || b.hi[d] >= n[d] || b.hi[d] < b.lo[d]) return false;
+        v += box_npts(b);
+    }
+    if (v != (long)n[0] * n[1] * n[2]) return false;
+    for (size_t a = 0; a < boxes.size(); ++a)
+        for (size_t b = a + 1; b < boxes.size(); ++b) {
+            bool apart = false;
+            for (int d = 0; d < 3; ++d)
+                apart = apart || boxes[a].hi[d] < boxes[b].lo[d] || boxes[b].hi[d] < boxes[a].lo[d];
+            if (!apart) return false;
+        }
+    return true;
+}
+
 } // namespace mc
 
 using namespace mc;
@@ -195,8 +216,8 @@ mc_level_t *mc_level_create(const int *n_cell, const double *prob_lo, const doub
         L->n_grown += box_npts(grow1(bx, 4)) * NVAR;
         vsum += box_npts(bx);
     }
-    if (vsum != ncell) {
-        set_error("mc_level_create: boxes do not tile the domain");
+    if (vsum != ncell || !tiles_domain(L->boxes, n_cell)) {
+        set_error("mc_level_create: boxes do not tile the domain exactly once");
         delete L;
         return nullptr;
     }
@@ -741,3 +762,172 @@ double mc_level_time(const mc_level_t *L) { return L->time; }
 double mc_level_dt(const mc_level_t *L) { return L->dt; }
 
 } // extern "C"
+
+// ------------------------------------------------------------------ batched advance
+// mc_advance_batch: Castro::advance + post_timestep over caller-owned FABs,
+// every box of the level in one launch per stage (the fused engine).  The
+// engine for the last geometry and box list is kept between calls.
+namespace mc {
+
+struct BatchFab {
+    int lo[3], hi[3];   // valid box
+    int flo[3], fhi[3]; // FAB extent
+    double *p;
+};
+
+// caller FABs <-> the engine's domain layout (component c of cell (i,j,k) at
+// c*nD + i + n0*(j + n1*k)); one thread per valid cell of the box list
+__global__ __launch_bounds__(kBlock) void k_batch_copy(const BatchFab *fabs, const long *bstart, int nbox, double *U,
+                                                     long nD, int n0, int n1, int to_domain)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= bstart[nbox]) return;
+    int lo = 0, hi = nbox - 1;  // box b with bstart[b] <= t < bstart[b+1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchFab F = fabs[lo];
+    const long lt = t - bstart[lo];
+    const int bx = F.hi[0] - F.lo[0] + 1, by = F.hi[1] - F.lo[1] + 1;
+    const int i = F.lo[0] + (int)(lt % bx), j = F.lo[1] + (int)((lt / bx) % by), k = F.lo[2] + (int)(lt / ((long)bx * by));
+    const long fx = F.fhi[0] - F.flo[0] + 1, fy = F.fhi[1] - F.flo[1] + 1, fz = F.fhi[2] - F.flo[2] + 1;
+    const long fn = fx * fy * fz;
+    const long f0 = (i - F.flo[0]) + fx * ((j - F.flo[1]) + fy * (long)(k - F.flo[2]));
+    const long c = (long)i + (long)n0 * (j + (long)n1 * k);
+    for (int n = 0; n < NVAR; ++n) {
+        if (to_domain) U[nD * n + c] = F.p[fn * n + f0];
+        else F.p[fn * n + f0] = U[nD * n + c];
+    }
+}
+
+struct BatchEngine {
+    int n_cell[3] = {0, 0, 0};
+    double prob_lo[3] = {0, 0, 0}, prob_hi[3] = {0, 0, 0};
+    int bc = -1;
+    std::vector<Box> boxes;
+    FusedLevel *F = nullptr;
+    BatchFab *fabs = nullptr;  // 2 * nbox entries: S_old FABs, then S_new FABs
+    long *bstart = nullptr;
+    std::vector<BatchFab> hfabs;
+};
+
+static BatchEngine g_batch;
+static std::mutex g_batch_mutex;
+
+static void batch_release()
+{
+    if (g_batch.F) fused_destroy(g_batch.F);
+    if (g_batch.fabs) (void)hipFree(g_batch.fabs);
+    if (g_batch.bstart) (void)hipFree(g_batch.bstart);
+    g_batch = BatchEngine();
+}
+
+static bool same_geometry(const BatchEngine &E, const int *n_cell, const double *prob_lo, const double *prob_hi,
+                          int bc, const std::vector<Box> &boxes)
+{
+    if (!E.F || E.bc != bc || E.boxes.size() != boxes.size()) return false;
+    for (int d = 0; d < 3; ++d)
+        if (E.n_cell[d] != n_cell[d] || E.prob_lo[d] != prob_lo[d] || E.prob_hi[d] != prob_hi[d]) return false;
+    for (size_t b = 0; b < boxes.size(); ++b)
+        for (int d = 0; d < 3; ++d)
+            if (E.boxes[b].lo[d] != boxes[b].lo[d] || E.boxes[b].hi[d] != boxes[b].hi[d]) return false;
+    return true;
+}
+
+} // namespace mc
+
+extern "C" int mc_advance_batch(const int *n_cell, const double *prob_lo, const double *prob_hi, int bc, int nbox,
+                                const mc_batch_box_t *work, double dt, double *dt_est)
+{
+    if (!eos_ready()) {
+        set_error("mc_advance_batch: EOS table not initialised");
+        return -1;
+    }
+    if (nbox <= 0 || !work || !n_cell || !prob_lo || !prob_hi || (bc != MC_BC_PERIODIC && bc != MC_BC_OUTFLOW) ||
+        !(dt > 0.0)) {
+        set_error("mc_advance_batch: bad arguments");
+        return -1;
+    }
+    std::vector<Box> boxes(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        const mc_batch_box_t &w = work[b];
+        if (!w.s_old || !w.s_new) {
+            set_error("mc_advance_batch: null FAB pointer");
+            return -1;
+        }
+        for (int d = 0; d < 3; ++d) {
+            boxes[b].lo[d] = w.lo[d];
+            boxes[b].hi[d] = w.hi[d];
+            if (w.old_lo[d] > w.lo[d] || w.old_hi[d] < w.hi[d] || w.new_lo[d] > w.lo[d] || w.new_hi[d] < w.hi[d]) {
+                set_error("mc_advance_batch: a FAB does not contain its box");
+                return -1;
+            }
+        }
+    }
+    if (!tiles_domain(boxes, n_cell)) {
+        set_error("mc_advance_batch: the boxes do not tile the domain exactly once");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    BatchEngine &E = g_batch;
+    hipStream_t s = cur_stream();
+    if (!same_geometry(E, n_cell, prob_lo, prob_hi, bc, boxes)) {
+        batch_release();
+        double dx[3];
+        for (int d = 0; d < 3; ++d) {
+            E.n_cell[d] = n_cell[d];
+            E.prob_lo[d] = prob_lo[d];
+            E.prob_hi[d] = prob_hi[d];
+            dx[d] = (prob_hi[d] - prob_lo[d]) / (double)n_cell[d];
+        }
+        E.bc = bc;
+        E.boxes = boxes;
+        std::vector<long> bs(nbox + 1, 0);
+        for (int b = 0; b < nbox; ++b) bs[b + 1] = bs[b] + box_npts(boxes[b]);
+        if (!(E.F = fused_create(boxes, n_cell, bc, dx)) ||
+            check_hip(hipMalloc(&E.fabs, sizeof(BatchFab) * 2 * nbox), "hipMalloc(batch fabs)") ||
+            check_hip(hipMalloc(&E.bstart, sizeof(long) * (nbox + 1)), "hipMalloc(batch starts)") ||
+            check_hip(hipMemcpy(E.bstart, bs.data(), sizeof(long) * (nbox + 1), hipMemcpyHostToDevice), "h2d")) {
+            batch_release();
+            return -1;
+        }
+        E.hfabs.resize(2 * nbox);
+    }
+    for (int b = 0; b < nbox; ++b) {
+        const mc_batch_box_t &w = work[b];
+        BatchFab &o = E.hfabs[b], &nw = E.hfabs[nbox + b];
+        for (int d = 0; d < 3; ++d) {
+            o.lo[d] = nw.lo[d] = w.lo[d];
+            o.hi[d] = nw.hi[d] = w.hi[d];
+            o.flo[d] = w.old_lo[d];
+            o.fhi[d] = w.old_hi[d];
+            nw.flo[d] = w.new_lo[d];
+            nw.fhi[d] = w.new_hi[d];
+        }
+        o.p = const_cast<double *>(w.s_old);
+        nw.p = w.s_new;
+    }
+    const long ncell = (long)n_cell[0] * n_cell[1] * n_cell[2];
+    const unsigned grid = (unsigned)((ncell + kBlock - 1) / kBlock);
+    double *U = fused_state(E.F, 0);
+    if (check_hip(hipMemcpyAsync(E.fabs, E.hfabs.data(), sizeof(BatchFab) * 2 * nbox, hipMemcpyHostToDevice, s),
+                  "h2d batch fabs"))
+        return -1;
+    hipLaunchKernelGGL(k_batch_copy, dim3(grid), dim3(kBlock), 0, s, E.fabs, E.bstart, nbox, U, ncell, n_cell[0],
+                       n_cell[1], 1);
+    if (fused_advance_dt(E.F, dt, s)) return -1;
+    hipLaunchKernelGGL(k_batch_copy, dim3(grid), dim3(kBlock), 0, s, E.fabs + nbox, E.bstart, nbox, U, ncell,
+                       n_cell[0], n_cell[1], 0);
+    double est = DBL_MAX;
+    if (fused_read_est(E.F, 0, &est, s)) return -1;  // synchronises the stream (the FAB table is reused)
+    if (dt_est) *dt_est = est * cfl;                  /* Castro.cpp:212-213 */
+    return check_hip(hipGetLastError(), "mc_advance_batch");
+}
+
+extern "C" void mc_advance_batch_release(void)
+{
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    batch_release();
+}

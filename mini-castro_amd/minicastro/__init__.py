@@ -9,4 +9,5 @@ from ._lib import lib, lib_path, build, load_table, table_path  # noqa: F401
 from .level import Level, MODE_PLAIN, MODE_FUSED, BC_PERIODIC, BC_OUTFLOW  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from .adjoint import Adjoint  # noqa: F401
+from .batch import advance_batch, work_item  # noqa: F401
 from . import configs  # noqa: F401

@@ -26,6 +26,14 @@ def build(jobs=8):
     subprocess.run(["make", "-s", "-j%d" % jobs, "-C", CSRC], check=True)
 
 
+class BatchBox(C.Structure):
+    """mc_batch_box_t: a valid box and the device FABs (pointer, lo, hi) of
+    S_old and S_new that hold it."""
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3),
+                ("s_old", C.c_void_p), ("old_lo", C.c_int * 3), ("old_hi", C.c_int * 3),
+                ("s_new", C.c_void_p), ("new_lo", C.c_int * 3), ("new_hi", C.c_int * 3)]
+
+
 # (name, restype, argtypes) for every exported symbol of include/mc_castro.h
 _I = C.c_int
 _D = C.c_double
@@ -85,6 +93,8 @@ SIGS = [
     ("mc_level_autotune", _I, [_vp, _I, _ip, _dp]),
     ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
     ("mc_level_blast_radius", _I, [_vp, _dp, _dp, _dp]),
+    ("mc_advance_batch", _I, [_ip, _dp, _dp, _I, _I, C.POINTER(BatchBox), _D, _dp]),
+    ("mc_advance_batch_release", None, []),
     ("mc_ensemble_create", _vp, [_I, _ip, _dp, _dp, _ip]),
     ("mc_ensemble_destroy", None, [_vp]),
     ("mc_ensemble_ninst", _I, [_vp]),

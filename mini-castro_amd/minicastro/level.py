@@ -73,6 +73,13 @@ class Level:
             raise RuntimeError("mc_level_est_time_step: %s" % self.L.mc_last_error().decode())
         return v
 
+    def initial_dt(self):
+        """Castro::computeInitialDt: init_shrink * estTimeStep."""
+        v = self.L.mc_level_initial_dt(self.h)
+        if v < 0:
+            raise RuntimeError("mc_level_initial_dt: %s" % self.L.mc_last_error().decode())
+        return v
+
     def advance(self, dt):
         check(self.L.mc_level_advance(self.h, dt), "mc_level_advance")
 

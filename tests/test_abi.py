@@ -56,4 +56,5 @@ def test_bad_box_lists_rejected():
     assert not mk([0, 0, 0, 8, 7, 7])            # outside the domain
     assert not mk([0, 0, 0, 3, 7, 7])            # does not tile the domain
     assert not mk([0, 0, 0, 7, 7, 7, 0, 0, 0, 7, 7, 7])  # overlap / double count
+    assert not mk([0, 0, 0, 7, 7, 3, 0, 0, 0, 7, 7, 3])  # overlap, volumes add up to the domain
     assert b"tile" in L.mc_last_error() or b"domain" in L.mc_last_error() or b"overlap" in L.mc_last_error()

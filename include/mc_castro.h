@@ -185,6 +185,12 @@ int mc_eos_init_generated(void);
 int mc_eos_eval_batch(int n, int mode, const double *rho, const double *T, const double *ev,
                       const double *abar, const double *zbar, double *out);
 
+/* EOS Newton solves (rho,e and rho,p inputs, every kernel of the library)
+   that ended after 100 iterations without converging.  The reference passes
+   them silently (Source/eos.F90:193,550) and so do the results here; the
+   count is the diagnostic.  Synchronises the device; reset != 0 clears it. */
+int mc_eos_newton_failures(unsigned long long *count, int reset);
+
 /* Problem constants used by initdata (Source/Prob.F90:11-15). */
 void mc_set_problem(double p_ambient, double dens_ambient, double exp_energy, double r_init, int nsub);
 

@@ -80,6 +80,11 @@ double *mc_ensemble_state_ptr(mc_ensemble_t *E, int m)
 int mc_ensemble_init_data(mc_ensemble_t *E, const double *p_ambient, const double *dens_ambient,
                           const double *exp_energy, const double *r_init, int nsub)
 {
+    for (int m = 0; m < E->ninst; ++m)
+        if (!cubic_cells(E->spec[m].dx)) {
+            set_error("mc_ensemble_init_data: We don't support dx != dy != dz");
+            return -1;
+        }
     void *prev = mc_get_stream();
     mc_set_stream(E->stream);
     for (int m = 0; m < E->ninst; ++m) {

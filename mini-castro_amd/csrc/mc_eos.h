@@ -38,7 +38,16 @@ struct EosTable {
     const double *node;  // IMAX*JMAX nodes, node (i,j) at (i-1) + IMAX*(j-1)
     const double *tg;    // JMAX entries
     const double *dg;    // IMAX entries
+    unsigned long long *newton_fail;  // Newton solves that ended unconverged (may be null)
 };
+
+// The reference lets a Newton solve that has not converged after
+// EOS_MAX_NEWTON iterations pass silently (Source/eos.F90:193,550); the
+// result is the same here, and the solve is also counted.
+__device__ __forceinline__ void eos_note_newton_failure(const EosTable &tab)
+{
+    if (tab.newton_fail) atomicAdd(tab.newton_fail, 1ull);
+}
 
 // Source/eos.F90:60-74,129-137 (the entropy constant sioncon is unused)
 namespace eosc {
@@ -323,6 +332,7 @@ __device__ __forceinline__ EosRE eos_re(const EosTable &tab, double rho, double 
         const double error = fabs((temp - temp_old) / temp_old);
         if (error < eosc::ttol) converged = true;
     }
+    if (!converged) eos_note_newton_failure(tab);
     EosRE r;
     r.T = temp;
     r.iat = s.iat;
@@ -369,6 +379,7 @@ __device__ __forceinline__ double eos_rp_e(const EosTable &tab, double rho, doub
         const double error = fabs((temp - temp_old) / temp_old);
         if (error < eosc::ttol) converged = true;
     }
+    if (!converged) eos_note_newton_failure(tab);
     if (T_out) *T_out = temp;
     return o.e;
 }

@@ -1,6 +1,8 @@
 // Library-internal helpers shared by the translation units.
 #pragma once
 
+#include <math.h>
+
 #include "mc_common.h"
 #include "mc_eos.h"
 
@@ -11,6 +13,13 @@ const EosTable &eos_table();
 bool eos_ready();
 void set_error(const char *msg);
 int check_hip(hipError_t e, const char *what);
+
+// Castro::initData's guard (Source/Castro.cpp:95-100): only dx = dy = dz
+inline bool cubic_cells(const double *dx)
+{
+    const double SMALL = 1.e-13;
+    return !((fabs(dx[0] - dx[1]) > SMALL * dx[0]) || (fabs(dx[0] - dx[2]) > SMALL * dx[0]));
+}
 
 // Asynchronous estdt over box bx of a state FAB on sbox: min-reduces the
 // CFL time step of every cell into *dt_dev (no synchronisation).

@@ -26,7 +26,8 @@ namespace mc {
 
 // ------------------------------------------------------------------ state
 static thread_local hipStream_t t_stream = nullptr;
-static EosTable g_eos = {nullptr, nullptr, nullptr};
+static EosTable g_eos = {nullptr, nullptr, nullptr, nullptr};
+static unsigned long long *g_eos_fail = nullptr;
 static double *g_eos_dev = nullptr;
 static std::mutex g_mu;
 static thread_local char t_err[512];
@@ -94,9 +95,15 @@ static int eos_upload(const double *tab)
     if (check_hip(hipMemcpy(g_eos_dev, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice),
                   "hipMemcpy(eos table)"))
         return -1;
+    if (!g_eos_fail) {
+        if (check_hip(hipMalloc(&g_eos_fail, sizeof(unsigned long long)), "hipMalloc(newton counter)") ||
+            check_hip(hipMemset(g_eos_fail, 0, sizeof(unsigned long long)), "hipMemset(newton counter)"))
+            return -1;
+    }
     g_eos.node = g_eos_dev;
     g_eos.tg = g_eos_dev + n_node;
     g_eos.dg = g_eos_dev + n_node + n_tg;
+    g_eos.newton_fail = g_eos_fail;
     return 0;
 }
 
@@ -758,8 +765,25 @@ void eos_finalize(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_eos_dev) (void)hipFree(g_eos_dev);
+    if (g_eos_fail) (void)hipFree(g_eos_fail);
     g_eos_dev = nullptr;
-    g_eos = EosTable{nullptr, nullptr, nullptr};
+    g_eos_fail = nullptr;
+    g_eos = EosTable{nullptr, nullptr, nullptr, nullptr};
+}
+
+int mc_eos_newton_failures(unsigned long long *count, int reset)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_eos_fail) {
+        set_error("mc_eos_newton_failures: EOS table not initialised");
+        return -1;
+    }
+    if (check_hip(hipDeviceSynchronize(), "newton counter sync")) return -1;
+    unsigned long long v = 0;
+    if (check_hip(hipMemcpy(&v, g_eos_fail, sizeof(v), hipMemcpyDeviceToHost), "newton counter d2h")) return -1;
+    if (count) *count = v;
+    if (reset && check_hip(hipMemset(g_eos_fail, 0, sizeof(v)), "newton counter reset")) return -1;
+    return 0;
 }
 
 // ---- Source/Castro_F.H:19-161 ----

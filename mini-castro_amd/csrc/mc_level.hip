@@ -465,6 +465,10 @@ int mc_level_sync(mc_level_t *L) { return check_hip(hipStreamSynchronize(L->stre
 int mc_level_init_data(mc_level_t *L, double p_ambient, double dens_ambient, double exp_energy, double r_init,
                        int nsub)
 {
+    if (!cubic_cells(L->dx)) {
+        set_error("mc_level_init_data: We don't support dx != dy != dz");
+        return -1;
+    }
     if (!eos_ready()) {
         set_error("mc_level_init_data: EOS table not initialised");
         return -1;

@@ -5,7 +5,7 @@ The compute path is the HIP library ``lib/libmccastro.so`` built from
 ctypes and adds the problem configurations.  There is no CPU fallback: a
 missing library raises.
 """
-from ._lib import lib, lib_path, build, load_table, table_path  # noqa: F401
+from ._lib import lib, lib_path, build, load_table, newton_failures, table_path  # noqa: F401
 from .level import Level, MODE_PLAIN, MODE_FUSED, BC_PERIODIC, BC_OUTFLOW  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from .adjoint import Adjoint  # noqa: F401

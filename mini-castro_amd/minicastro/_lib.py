@@ -68,6 +68,7 @@ SIGS = [
     ("mc_eos_init_arrays", _I, [_dp]),
     ("mc_eos_init_generated", _I, []),
     ("mc_eos_eval_batch", _I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp]),
+    ("mc_eos_newton_failures", _I, [C.POINTER(C.c_ulonglong), _I]),
     ("mc_set_problem", None, [_D, _D, _D, _D, _I]),
     ("mc_level_create", _vp, [_ip, _dp, _dp, _I, _I, _ip]),
     ("mc_level_destroy", None, [_vp]),
@@ -138,6 +139,13 @@ def check(rc, what):
 
 
 _table_loaded = False
+
+
+def newton_failures(reset=False):
+    """EOS Newton solves that ended unconverged since the last reset."""
+    v = C.c_ulonglong(0)
+    check(lib().mc_eos_newton_failures(C.byref(v), int(bool(reset))), "mc_eos_newton_failures")
+    return int(v.value)
 
 
 def load_table(path=None):

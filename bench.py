@@ -200,9 +200,11 @@ def ensemble_bench(steps=50, n_inst=24):
 
 def adjoint_bench(reps=3):
     """Discrete adjoint on the 64^3 Sedov adjoint workload, 10 steps: time of
-    the checkpointing forward pass and of the adjoint pass (recompute + the
-    reversed stages), and the central-difference check of two smooth random
-    directions (eps 1e-6 and 1e-7)."""
+    the checkpointing forward pass and of the adjoint pass (the reversed
+    stages, reading each step's intermediates as the forward pass kept them),
+    the same with every step recomputed from its checkpoint instead, and the
+    central-difference check of two smooth random directions (eps 1e-6 and
+    1e-7)."""
     import numpy as np
     import torch
     from minicastro import configs
@@ -214,17 +216,28 @@ def adjoint_bench(reps=3):
     out, gin = torch.empty_like(U), torch.empty_like(U)
     seed = torch.zeros_like(U)
     seed[0] = 2.0 * P.mask
-    fwd, adj = [], []
-    for r in range(reps + 1):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        P.adj.forward(U.data_ptr(), out.data_ptr(), cfg["steps"], P.dts)
-        t1 = time.perf_counter()
-        P.adj.backward(seed.data_ptr(), gin.data_ptr())
-        t2 = time.perf_counter()
-        if r:
-            fwd.append(t1 - t0)
-            adj.append(t2 - t1)
+    def timed():
+        fwd, adj = [], []
+        for r in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            P.adj.forward(U.data_ptr(), out.data_ptr(), cfg["steps"], P.dts)
+            t1 = time.perf_counter()
+            P.adj.backward(seed.data_ptr(), gin.data_ptr())
+            t2 = time.perf_counter()
+            if r:
+                fwd.append(t1 - t0)
+                adj.append(t2 - t1)
+        return float(np.mean(fwd)) * 1e3, float(np.mean(adj)) * 1e3
+
+    kept = P.adj.keeps_intermediates
+    f, a_ = timed()
+    rec = None
+    if kept:
+        P.adj.keep_intermediates(False)
+        fr, ar = timed()
+        P.adj.keep_intermediates(True)
+        rec = {"forward_ms": fr, "adjoint_ms": ar, "adjoint_over_forward": ar / fr}
     rng = np.random.default_rng(7)
     errs = []
     for q in range(2):
@@ -236,9 +249,9 @@ def adjoint_bench(reps=3):
             fd = (P.J(P.rho0 + eps * v, P.E + eps * vE) - P.J(P.rho0 - eps * v, P.E - eps * vE)) / (2 * eps)
             e.append(abs(fd - ad) / abs(ad))
         errs.append(e)
-    f, a_ = float(np.mean(fwd)) * 1e3, float(np.mean(adj)) * 1e3
     return {"workload": cfg["name"], "steps": cfg["steps"], "forward_ms": f, "adjoint_ms": a_,
-            "adjoint_over_forward": a_ / f, "J": J, "dJ_dE": g_E,
+            "adjoint_over_forward": a_ / f, "intermediates": "kept" if kept else "recomputed",
+            "recompute_mode": rec, "J": J, "dJ_dE": g_E,
             "fd_rel_err_smooth_dirs": {"eps_1e-6": [e[0] for e in errs], "eps_1e-7": [e[1] for e in errs]}}
 
 

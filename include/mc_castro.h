@@ -324,6 +324,14 @@ int mc_adjoint_forward(mc_adjoint_t *A, const double *U0, double *U_out, int nst
 /* Ub0 = (dU_N/dU0)^T Ub_final for the last forward pass */
 int mc_adjoint_backward(mc_adjoint_t *A, const double *Ub_final, double *Ub0);
 int mc_adjoint_steps(const mc_adjoint_t *A);
+/* The reverse pass needs every step's intermediates (primitives, traces,
+   fluxes).  By default, when max_steps sets of them fit in half the free
+   device memory, the forward pass keeps them (one set per step) and the
+   reverse pass reads them; otherwise each step is recomputed from its
+   checkpoint.  keep = 0 selects recomputation, keep != 0 keeping (an error
+   if the sets do not fit).  Either change drops the last forward pass. */
+int mc_adjoint_keep_intermediates(mc_adjoint_t *A, int keep);
+int mc_adjoint_keeps_intermediates(const mc_adjoint_t *A);
 
 #ifdef __cplusplus
 }

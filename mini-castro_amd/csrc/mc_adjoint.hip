@@ -681,7 +681,41 @@ struct mc_adjoint {
     long abuf_n = 0;
     double *ub0 = nullptr, *ub1 = nullptr;  // NHB x nD each
     ACtx a;
+    // Kept intermediates: step n of the forward pass writes its primitives,
+    // traces and fluxes into sets[n], and the reverse pass reads them there
+    // instead of recomputing the step.  Empty: recompute mode.
+    std::vector<FusedArrays> sets;
+    FusedArrays own;  // the engine's own set
 };
+
+static void adjoint_drop_sets(mc_adjoint *A)
+{
+    fused_set_intermediates(A->F, A->own);
+    for (auto &S : A->sets) fused_free_intermediates(S);
+    A->sets.clear();
+    A->nsteps = 0;  // a kept forward pass is gone
+}
+
+// One set per step when max_steps sets fit in half of the free device memory.
+static int adjoint_keep_sets(mc_adjoint *A, bool report)
+{
+    if (!A->sets.empty()) return 0;
+    size_t free_b = 0, total_b = 0;
+    if (check_hip(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return -1;
+    const size_t need = fused_intermediate_bytes(A->F) * (size_t)A->max_steps;
+    if (need > free_b / 2) {
+        if (report) set_error("mc_adjoint: the intermediates of max_steps steps do not fit in half the free device memory");
+        return -1;
+    }
+    A->sets.resize(A->max_steps);
+    for (auto &S : A->sets)
+        if (fused_alloc_intermediates(A->F, S)) {
+            adjoint_drop_sets(A);
+            return -1;
+        }
+    A->nsteps = 0;
+    return 0;
+}
 
 static void launch_adj(mc_adjoint *A, int stage, hipStream_t s)
 {
@@ -746,6 +780,8 @@ mc_adjoint_t *mc_adjoint_create(const int *n_cell, const double *prob_lo, const 
         mc_adjoint_destroy(A);
         return nullptr;
     }
+    A->own = fused_get_intermediates(A->F);
+    (void)adjoint_keep_sets(A, false);  // else recompute mode
     double *q = A->abuf;
     ACtx &a = A->a;
     a.U1b = q;
@@ -776,7 +812,11 @@ mc_adjoint_t *mc_adjoint_create(const int *n_cell, const double *prob_lo, const 
 void mc_adjoint_destroy(mc_adjoint_t *A)
 {
     if (!A) return;
-    if (A->F) fused_destroy(A->F);
+    if (A->F) {
+        (void)hipDeviceSynchronize();
+        adjoint_drop_sets(A);
+        fused_destroy(A->F);
+    }
     (void)hipFree(A->ckpt);
     (void)hipFree(A->abuf);
     (void)hipFree(A->ub0);
@@ -818,9 +858,11 @@ int mc_adjoint_forward(mc_adjoint_t *A, const double *U0, double *U_out, int nst
         if (check_hip(hipMemcpyAsync(A->ckpt + (size_t)n * NVAR * A->nD, U, bytes, hipMemcpyDeviceToDevice, s),
                       "checkpoint"))
             return -1;
+        if (!A->sets.empty()) fused_set_intermediates(F, A->sets[n]);
         if (fused_advance_dt(F, dt, s)) return -1;
         if (compute_dt && fused_read_est(F, 0, &est, s)) return -1;
     }
+    if (!A->sets.empty()) fused_set_intermediates(F, A->own);
     A->nsteps = nsteps;
     if (check_hip(hipMemcpyAsync(U_out, U, bytes, hipMemcpyDeviceToDevice, s), "adjoint forward")) return -1;
     return check_hip(hipStreamSynchronize(s), "adjoint forward");
@@ -845,8 +887,14 @@ int mc_adjoint_backward(mc_adjoint_t *A, const double *Ub_final, double *Ub0)
     double *U = fused_state(F, 0);
     for (int n = A->nsteps - 1; n >= 0; --n) {
         const double *ck = A->ckpt + (size_t)n * NVAR * nD;
-        if (check_hip(hipMemcpyAsync(U, ck, bytes, hipMemcpyDeviceToDevice, s), "recompute")) return -1;
-        if (fused_recompute_dt(F, A->dts[n], s)) return -1;  // recompute the step's intermediates
+        if (!A->sets.empty()) {
+            // the step's intermediates as the forward pass left them
+            fused_set_intermediates(F, A->sets[n]);
+            fill(F->ctx.dt_cur, 1, A->dts[n], s);
+        } else {
+            if (check_hip(hipMemcpyAsync(U, ck, bytes, hipMemcpyDeviceToDevice, s), "recompute")) return -1;
+            if (fused_recompute_dt(F, A->dts[n], s)) return -1;  // recompute the step's intermediates
+        }
         fill(A->abuf, A->abuf_n, 0.0, s);
         A->a.Uold = ck;
         A->a.Ub = cur;
@@ -861,9 +909,29 @@ int mc_adjoint_backward(mc_adjoint_t *A, const double *Ub_final, double *Ub0)
         if (check_hip(hipGetLastError(), "adjoint step")) return -1;
         std::swap(cur, nxt);
     }
+    if (!A->sets.empty()) fused_set_intermediates(F, A->own);
     hipLaunchKernelGGL(A_unseed, dim3(gb), dim3(256), 0, s, cur, Ub0, nD);
     return check_hip(hipStreamSynchronize(s), "adjoint backward");
 }
+
+// keep != 0: keep every step's intermediates (allocated now; fails if
+// max_steps sets do not fit in half the free device memory); 0: recompute
+// each step from its checkpoint.  Either change drops the last forward pass.
+int mc_adjoint_keep_intermediates(mc_adjoint_t *A, int keep)
+{
+    if (!A) {
+        set_error("mc_adjoint_keep_intermediates: null handle");
+        return -1;
+    }
+    (void)hipDeviceSynchronize();
+    if (!keep) {
+        adjoint_drop_sets(A);
+        return 0;
+    }
+    return adjoint_keep_sets(A, true);
+}
+
+int mc_adjoint_keeps_intermediates(const mc_adjoint_t *A) { return A && !A->sets.empty() ? 1 : 0; }
 
 int mc_adjoint_steps(const mc_adjoint_t *A) { return A ? A->nsteps : -1; }
 

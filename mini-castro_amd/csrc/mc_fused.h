@@ -40,4 +40,17 @@ void fused_get_tiling(FusedLevel *F, int stage, int *bxyz);
 int fused_run_stage(FusedLevel *F, int stage, hipStream_t s);
 int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *best_ms);
 
+// The arrays a step writes besides the state (primitives, flattening,
+// traces, fluxes).  A caller may give the engine another set to write into,
+// to keep a step's intermediates instead of recomputing them later.
+struct FusedArrays {
+    double *U1, *Q, *FL, *DV;
+    double *SM[3], *SP[3], *FD[3], *FT[6], *FN[3];
+};
+size_t fused_intermediate_bytes(const FusedLevel *F);
+int fused_alloc_intermediates(const FusedLevel *F, FusedArrays &S);  // all null on failure
+void fused_free_intermediates(FusedArrays &S);
+FusedArrays fused_get_intermediates(const FusedLevel *F);
+void fused_set_intermediates(FusedLevel *F, const FusedArrays &S);  // drops a captured graph
+
 } // namespace mc

@@ -745,6 +745,91 @@ int fused_export(FusedLevel *F, double *arena, hipStream_t s)
     return check_hip(hipGetLastError(), "fused_export");
 }
 
+// ---- intermediate sets
+template <class Fn>
+static void for_each_intermediate(FusedArrays &S, long totD, long totE, Fn f)
+{
+    f(S.U1, NVAR * totD);
+    f(S.Q, NFQ * totD);
+    f(S.FL, totE);
+    f(S.DV, totE);
+    for (int d = 0; d < 3; ++d) {
+        f(S.SM[d], NFS * totE);
+        f(S.SP[d], NFS * totE);
+        f(S.FD[d], NFF * totE);
+        f(S.FN[d], NFN * totE);
+    }
+    for (int c = 0; c < 6; ++c) f(S.FT[c], NFF * totE);
+}
+
+size_t fused_intermediate_bytes(const FusedLevel *F)
+{
+    FusedArrays S{};
+    size_t n = 0;
+    for_each_intermediate(S, F->totD, F->totE, [&](double *&, long c) { n += sizeof(double) * std::max(c, 1L); });
+    return n;
+}
+
+int fused_alloc_intermediates(const FusedLevel *F, FusedArrays &S)
+{
+    S = FusedArrays{};
+    bool ok = true;
+    for_each_intermediate(S, F->totD, F->totE, [&](double *&p, long c) {
+        if (ok && check_hip(hipMalloc(&p, sizeof(double) * std::max(c, 1L)), "hipMalloc(intermediates)")) ok = false;
+    });
+    if (!ok) fused_free_intermediates(S);
+    return ok ? 0 : -1;
+}
+
+void fused_free_intermediates(FusedArrays &S)
+{
+    FusedArrays T = S;
+    for_each_intermediate(T, 0, 0, [&](double *&p, long) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    });
+    S = FusedArrays{};
+}
+
+FusedArrays fused_get_intermediates(const FusedLevel *F)
+{
+    const Ctx &x = F->ctx;
+    FusedArrays S;
+    S.U1 = x.U1;
+    S.Q = x.Q;
+    S.FL = x.FL;
+    S.DV = x.DV;
+    for (int d = 0; d < 3; ++d) {
+        S.SM[d] = x.SM[d];
+        S.SP[d] = x.SP[d];
+        S.FD[d] = x.FD[d];
+        S.FN[d] = x.FN[d];
+    }
+    for (int c = 0; c < 6; ++c) S.FT[c] = x.FT[c];
+    return S;
+}
+
+void fused_set_intermediates(FusedLevel *F, const FusedArrays &S)
+{
+    Ctx &x = F->ctx;
+    x.U1 = S.U1;
+    x.Q = S.Q;
+    x.FL = S.FL;
+    x.DV = S.DV;
+    for (int d = 0; d < 3; ++d) {
+        x.SM[d] = S.SM[d];
+        x.SP[d] = S.SP[d];
+        x.FD[d] = S.FD[d];
+        x.FN[d] = S.FN[d];
+    }
+    for (int c = 0; c < 6; ++c) x.FT[c] = S.FT[c];
+    // a captured graph holds the old pointers
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    F->gexec = nullptr;
+    F->graph = nullptr;
+}
+
 double *fused_state(FusedLevel *F, int m) { return F->ctx.U + (long)NVAR * F->hinst[m].offD; }
 long fused_ncell(FusedLevel *F, int m) { return F->hinst[m].nD; }
 int fused_ninst(FusedLevel *F) { return F->ninst; }

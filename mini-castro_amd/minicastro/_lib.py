@@ -111,6 +111,8 @@ SIGS = [
     ("mc_adjoint_forward", _I, [_vp, _vp, _vp, _I, _dp, _I]),
     ("mc_adjoint_backward", _I, [_vp, _vp, _vp]),
     ("mc_adjoint_steps", _I, [_vp]),
+    ("mc_adjoint_keep_intermediates", _I, [_vp, _I]),
+    ("mc_adjoint_keeps_intermediates", _I, [_vp]),
 ]
 
 _lib = None

@@ -52,3 +52,14 @@ class Adjoint:
         adjoint of its final state."""
         check(self.L.mc_adjoint_backward(self.h, C.c_void_p(ub_final_ptr), C.c_void_p(ub0_ptr)),
               "mc_adjoint_backward")
+
+    @property
+    def keeps_intermediates(self):
+        """True when the forward pass keeps every step's intermediates (the
+        reverse pass then reads them instead of recomputing each step)."""
+        return bool(self.L.mc_adjoint_keeps_intermediates(self.h))
+
+    def keep_intermediates(self, keep):
+        """Select keeping (True) or recomputing (False) the intermediates;
+        drops the last forward pass."""
+        check(self.L.mc_adjoint_keep_intermediates(self.h, 1 if keep else 0), "mc_adjoint_keep_intermediates")

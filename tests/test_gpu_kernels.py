@@ -351,3 +351,35 @@ def test_eos_batch_bitwise():
         if mode == 1:
             np.testing.assert_array_equal(o[:, 2], [r.e for r in rr])
             np.testing.assert_array_equal(o[:, 7], [r.dedT for r in rr])
+
+
+def test_eos_hash_on_grid_lines():
+    """The table index hash takes log10 (ocml on the GPU, glibc in the
+    oracle).  Inputs on every grid line of T and of ye*rho, and one ulp to
+    either side, are where the two could pick different table cells: the
+    indices (and the results) must still agree bit for bit."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    mc.load_table(table_bin())
+    oracle_table()
+    t, d = orc.eos_grid()
+    near = lambda g: np.concatenate([np.nextafter(g, 0.0), g, np.nextafter(g, np.inf)])
+    T_pts = near(t[:-1])
+    din_pts = near(d[:-1])
+    rho = np.concatenate([np.full(T_pts.size, 2.0), 2.0 * din_pts])   # ye = 1/2: ye*rho = din exactly
+    T = np.concatenate([T_pts, np.full(din_pts.size, 1.0e7)])
+    n = rho.size
+    abar, zbar = np.full(n, 4.0), np.full(n, 2.0)
+    ref = [orc.eos(orc.EOS_RT, r_, t_, 4.0, 2.0) for r_, t_ in zip(rho, T)]
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).cuda()
+    P = lambda x: C.cast(x.data_ptr(), C.POINTER(C.c_double))
+    out = torch.zeros(10 * n, dtype=torch.float64, device="cuda")
+    args = [dev(rho), dev(T), dev(np.zeros(n)), dev(abar), dev(zbar)]
+    assert mc.lib().mc_eos_eval_batch(n, 1, *[P(a) for a in args], P(out)) == 0
+    torch.cuda.synchronize()
+    o = out.cpu().numpy().reshape(n, 10)
+    np.testing.assert_array_equal(o[:, 8], [r.iat for r in ref])
+    np.testing.assert_array_equal(o[:, 9], [r.jat for r in ref])
+    np.testing.assert_array_equal(o[:, 1], [r.p for r in ref])
+    np.testing.assert_array_equal(o[:, 2], [r.e for r in ref])

@@ -31,8 +31,8 @@ def gpu_level(cfg, boxes=None, mode=mc.MODE_PLAIN):
     return L
 
 
-def run_pair(cfg, steps, boxes=None, mode=mc.MODE_PLAIN, tol=0.0):
-    O = oracle_level(cfg, nthreads=8)
+def run_pair(cfg, steps, boxes=None, mode=mc.MODE_PLAIN, tol=0.0, nthreads=8):
+    O = oracle_level(cfg, nthreads=nthreads)
     G = gpu_level(cfg, boxes, mode)
     np.testing.assert_array_equal(G.state(), O.state())
     for s in range(steps):
@@ -75,6 +75,26 @@ def test_config2_sedov128_8x64_20_steps():
         G.coarse_step()
     F.run(cfg["steps"] - 2, use_graph=True)
     assert (F.steps, F.dt, F.time) == (G.steps, G.dt, G.time) == (cfg["steps"], G.dt, G.time)
+    np.testing.assert_array_equal(F.state(), G.state())
+
+
+def test_config3_sedov256_uneven64():
+    """Config 3, the benchmark workload: 256^3 in the 64 seeded uneven boxes.
+    The per-box GPU path equals the oracle bit for bit over the first step
+    (about a minute of oracle time on 16 host threads), and the batched path
+    equals the per-box path bit for bit, dt included, over the first 12 steps
+    (the benchmark times steps 3-12)."""
+    import os
+    cfg = configs.config(3)
+    G, O = run_pair(cfg, 1, nthreads=max(1, min(16, os.cpu_count() or 1)))
+    del O
+    F = gpu_level(cfg, mode=mc.MODE_FUSED)
+    F.run(1, use_graph=False)
+    np.testing.assert_array_equal(F.state(), G.state())
+    for _ in range(11):
+        G.coarse_step()
+    F.run(11, use_graph=True)
+    assert (F.steps, F.dt, F.time) == (G.steps, G.dt, G.time)
     np.testing.assert_array_equal(F.state(), G.state())
 
 

@@ -60,6 +60,19 @@ __device__ __forceinline__ void aadd(gdouble *p, double v)
         }
     }
 }
+// plain read-add-write, for an entry no other thread of the launch touches
+__device__ __forceinline__ void radd(gdouble *p, double v)
+{
+    if (v != 0.0) *p = *p + v;
+}
+// add into an adjoint entry: SOLE = this thread is the entry's only writer
+// in the launch (no atomic needed)
+template <bool SOLE>
+__device__ __forceinline__ void badd(gdouble *p, double v)
+{
+    if (SOLE) radd(p, v);
+    else aadd(p, v);
+}
 __device__ __forceinline__ gdouble *EAp(double *a, const FInst &I, int C, int c, idx_t idx)
 {
     return &at(a + ((long)C * I.offE + (long)c * I.nE), idx);
@@ -184,25 +197,27 @@ __device__ __forceinline__ QStateHB from_faceb(const FaceHB &f, int d)
     return o;
 }
 
+template <bool SOLE = false>
 __device__ __forceinline__ void add_stateb(double *S, const FInst &I, idx_t e, const QInHB &b)
 {
-    aadd(EAp(S, I, NSB, SB_RHO, e), b.rho);
-    aadd(EAp(S, I, NSB, SB_U, e), b.u);
-    aadd(EAp(S, I, NSB, SB_V, e), b.v);
-    aadd(EAp(S, I, NSB, SB_W, e), b.w);
-    aadd(EAp(S, I, NSB, SB_P, e), b.p);
-    aadd(EAp(S, I, NSB, SB_RE, e), b.rhoe);
+    badd<SOLE>(EAp(S, I, NSB, SB_RHO, e), b.rho);
+    badd<SOLE>(EAp(S, I, NSB, SB_U, e), b.u);
+    badd<SOLE>(EAp(S, I, NSB, SB_V, e), b.v);
+    badd<SOLE>(EAp(S, I, NSB, SB_W, e), b.w);
+    badd<SOLE>(EAp(S, I, NSB, SB_P, e), b.p);
+    badd<SOLE>(EAp(S, I, NSB, SB_RE, e), b.rhoe);
 }
 
+template <bool SOLE = false>
 __device__ __forceinline__ void add_faceb(double *S, const FInst &I, idx_t e, int d, const FaceHB &f)
 {
     const int un = SB_U + d, ut = d == 0 ? SB_V : SB_U, utt = d == 2 ? SB_V : SB_W;
-    aadd(EAp(S, I, NSB, SB_RHO, e), f.rho);
-    aadd(EAp(S, I, NSB, un, e), f.un);
-    aadd(EAp(S, I, NSB, ut, e), f.ut);
-    aadd(EAp(S, I, NSB, utt, e), f.utt);
-    aadd(EAp(S, I, NSB, SB_P, e), f.p);
-    aadd(EAp(S, I, NSB, SB_RE, e), f.rhoe);
+    badd<SOLE>(EAp(S, I, NSB, SB_RHO, e), f.rho);
+    badd<SOLE>(EAp(S, I, NSB, un, e), f.un);
+    badd<SOLE>(EAp(S, I, NSB, ut, e), f.ut);
+    badd<SOLE>(EAp(S, I, NSB, utt, e), f.utt);
+    badd<SOLE>(EAp(S, I, NSB, SB_P, e), f.p);
+    badd<SOLE>(EAp(S, I, NSB, SB_RE, e), f.rhoe);
 }
 
 // adjoint of a transverse-stage flux entry (URHO..UEDEN + Godunov p, u)
@@ -347,7 +362,8 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
                f3lb[5] = {0, 0, 0, 0, 0}, gb[8] = {0, 0, 0, 0, 0, 0, 0, 0}, gamcb = 0.0;
         trans2_hydro_b(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m, gamc, h2, h3, ob, lqb,
                        f2rb, f2lb, f3rb, f3lb, gb, gamcb);
-        add_stateb(s == 0 ? a.SMb[d] : a.SPb[d], I, e, lqb);
+        // the d-face states at e are this thread's alone in A_final
+        add_stateb<true>(s == 0 ? a.SMb[d] : a.SPb[d], I, e, lqb);
         add_tfluxb(a.FTb[p2], I, r2[s], f2rb, gb[0], gb[2]);
         add_tfluxb(a.FTb[p2], I, l2[s], f2lb, gb[1], gb[3]);
         add_tfluxb(a.FTb[p3], I, r3[s], f3rb, gb[4], gb[6]);
@@ -428,8 +444,9 @@ __global__ __launch_bounds__(256) void A_flux1(Plan p, Ctx x, ACtx a)
     const RiemannHB rb = load_tfluxb(a.FDb[d], I, e, d);
     FaceHB Lb = {0, 0, 0, 0, 0, 0}, Rb = {0, 0, 0, 0, 0, 0};
     face_riemann_hb(x, a, I, d, i, j, k, L, R, rb, Lb, Rb);
-    add_faceb(a.SMb[d], I, e, d, Lb);
-    add_faceb(a.SPb[d], I, e, d, Rb);
+    // one thread per d-face: the states at e are this thread's alone here
+    add_faceb<true>(a.SMb[d], I, e, d, Lb);
+    add_faceb<true>(a.SPb[d], I, e, d, Rb);
 }
 
 // ------------------------------------------------------------------ A_trace

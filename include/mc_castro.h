@@ -191,7 +191,10 @@ int mc_eos_eval_batch(int n, int mode, const double *rho, const double *T, const
    count is the diagnostic.  Synchronises the device; reset != 0 clears it. */
 int mc_eos_newton_failures(unsigned long long *count, int reset);
 
-/* Problem constants used by initdata (Source/Prob.F90:11-15). */
+/* Problem constants used by initdata (Source/Prob.F90:11-15): one
+   process-wide set, like the reference's probdata module.  The level and
+   ensemble init_data calls take their own constants and leave this set
+   alone, so drivers initialised from different host threads do not race. */
 void mc_set_problem(double p_ambient, double dens_ambient, double exp_energy, double r_init, int nsub);
 
 /* ---- level driver: one AMR level, a list of boxes tiling the domain ----

@@ -90,8 +90,8 @@ int mc_ensemble_init_data(mc_ensemble_t *E, const double *p_ambient, const doubl
     for (int m = 0; m < E->ninst; ++m) {
         const FusedInstanceSpec &s = E->spec[m];
         const int lo[3] = {0, 0, 0}, hi[3] = {s.n[0] - 1, s.n[1] - 1, s.n[2] - 1};
-        mc_set_problem(p_ambient[m], dens_ambient[m], exp_energy[m], r_init[m], nsub);
-        initdata(lo, hi, fused_state(E->F, m), lo, hi, s.dx, &E->prob_lo[3 * m], &E->prob_hi[3 * m]);
+        const ProblemConsts pc = {p_ambient[m], dens_ambient[m], exp_energy[m], r_init[m], nsub};
+        initdata_with(pc, lo, hi, fused_state(E->F, m), lo, hi, s.dx, &E->prob_lo[3 * m], &E->prob_hi[3 * m]);
         if (fused_set_clock(E->F, m, 0.0, 0.0, 0, E->stream)) break;
     }
     mc_set_stream(prev);

@@ -950,21 +950,33 @@ void trace_ppm(const int *lo, const int *hi, const int *vlo, const int *vhi, con
 void initdata(const int *lo, const int *hi, double *state, const int *state_lo, const int *state_hi,
               const double *dx, const double *problo, const double *probhi)
 {
-    REQUIRE_EOS();
+    const ProblemConsts pc = {g_p_ambient, g_dens_ambient, g_exp_energy, g_r_init, g_nsub};
+    initdata_with(pc, lo, hi, state, state_lo, state_hi, dx, problo, probhi);
+}
+} // extern "C"
+
+namespace mc {
+void initdata_with(const ProblemConsts &pc, const int *lo, const int *hi, double *state, const int *state_lo,
+                   const int *state_hi, const double *dx, const double *problo, const double *probhi)
+{
+    if (!eos_ready()) {
+        set_error("initdata: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return;
+    }
     double *e_dev = scratch_dev(1);
     if (!e_dev) return;
-    hipLaunchKernelGGL(k_prob_eamb, dim3(1), dim3(64), 0, t_stream, g_dens_ambient, g_p_ambient, e_dev, g_eos);
+    hipLaunchKernelGGL(k_prob_eamb, dim3(1), dim3(64), 0, t_stream, pc.dens_ambient, pc.p_ambient, e_dev, g_eos);
     double e_amb = 0;
     if (check_hip(hipMemcpyAsync(&e_amb, e_dev, sizeof(double), hipMemcpyDeviceToHost, t_stream), "initdata d2h"))
         return;
     if (check_hip(hipStreamSynchronize(t_stream), "initdata sync")) return;
     ProbDev P;
-    P.dens_ambient = g_dens_ambient;
+    P.dens_ambient = pc.dens_ambient;
     P.e_ambient = e_amb;
-    const double vctr = FOUR3RD * M_PI_D * (g_r_init * g_r_init * g_r_init);
-    P.e_exp = g_exp_energy / vctr / g_dens_ambient;
-    P.r_init = g_r_init;
-    P.nsub = g_nsub;
+    const double vctr = FOUR3RD * M_PI_D * (pc.r_init * pc.r_init * pc.r_init);
+    P.e_exp = pc.exp_energy / vctr / pc.dens_ambient;
+    P.r_init = pc.r_init;
+    P.nsub = pc.nsub;
     for (int d = 0; d < 3; ++d) {
         P.problo[d] = problo[d];
         P.dx[d] = dx[d];
@@ -973,6 +985,9 @@ void initdata(const int *lo, const int *hi, double *state, const int *state_lo, 
     BoxIter it = make_iter(make_box(lo, hi));
     hipLaunchKernelGGL(k_initdata, dim3(grid_for(it.n)), dim3(kBlock), 0, t_stream, it, ARR(state), P);
 }
+} // namespace mc
+
+extern "C" {
 
 void denerror(const int *lo, const int *hi, int8_t *tag, const int *tag_lo, const int *tag_hi, const double *den,
               const int *den_lo, const int *den_hi, const int8_t tagval, const int8_t clearval)

@@ -473,13 +473,13 @@ int mc_level_init_data(mc_level_t *L, double p_ambient, double dens_ambient, dou
         set_error("mc_level_init_data: EOS table not initialised");
         return -1;
     }
-    mc_set_problem(p_ambient, dens_ambient, exp_energy, r_init, nsub);
+    const ProblemConsts pc = {p_ambient, dens_ambient, exp_energy, r_init, nsub};
     void *prev = mc_get_stream();
     mc_set_stream(L->stream);
     fill_const(L->stream, L->S_new, L->n_valid, 0.0); /* S_new.setVal(0.), Castro.cpp:93 */
     for (size_t b = 0; b < L->boxes.size(); ++b) {
         const Box &bx = L->boxes[b];
-        initdata(bx.lo, bx.hi, L->S_new + L->off_valid[b], bx.lo, bx.hi, L->dx, L->prob_lo, L->prob_hi);
+        initdata_with(pc, bx.lo, bx.hi, L->S_new + L->off_valid[b], bx.lo, bx.hi, L->dx, L->prob_lo, L->prob_hi);
     }
     mc_set_stream(prev);
     L->steps = 0;

@@ -61,6 +61,21 @@ def test_uneven_boxes_bitwise():
     run_pair(cfg, 3, boxes=boxes)
 
 
+def test_ragged_thin_boxes_bitwise():
+    """Boxes one to three cells thick: every stencil of the step reaches
+    across several boxes, so the ghost fill and the box algebra are
+    exercised at their limits.  Per-box path == oracle, batched == per-box."""
+    cfg = small_cfg(n=12, bc=1, r_init=0.2)
+    cx, cy, cz = [0, 1, 3, 12], [0, 2, 5, 12], [0, 3, 4, 12]
+    boxes = np.array([[cx[a], cy[b], cz[c], cx[a + 1] - 1, cy[b + 1] - 1, cz[c + 1] - 1]
+                      for c in range(3) for b in range(3) for a in range(3)], dtype=np.int32)
+    G, _ = run_pair(cfg, 3, boxes=boxes)
+    F = gpu_level(cfg, boxes=boxes, mode=mc.MODE_FUSED)
+    F.run(3, use_graph=False)
+    assert (F.dt, F.time) == (G.dt, G.time)
+    np.testing.assert_array_equal(F.state(), G.state())
+
+
 def test_config2_sedov128_8x64_20_steps():
     """Config 2: 128^3 in eight 64^3 boxes, 20 steps.  The per-box GPU path
     equals the oracle bit for bit over the first two steps (the oracle is

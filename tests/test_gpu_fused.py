@@ -124,6 +124,25 @@ def test_ensemble_matches_sequential_runs():
         np.testing.assert_array_equal(ens.state(m), L.state(), err_msg="instance %d" % m)
 
 
+def test_config4_ensemble_24_instances_50_steps():
+    """Config 4: the 24 seeded Sedov instances (sides 32-96, energies, radii
+    and densities drawn per instance), 50 steps each with its own dt, all in
+    one launch per stage.  Every instance's clock and state equal the same
+    instance advanced alone by the per-box path, bit for bit."""
+    insts = configs.ensemble(24, seed=7)
+    ens = mc.Ensemble(insts)
+    ens.init_data()
+    ens.run(50, use_graph=True)
+    for m, d in enumerate(insts):
+        L = gpu_level(d, mode=mc.MODE_PLAIN)
+        L.run(50)
+        dt, t, steps = ens.clock(m)
+        assert steps == L.steps == 50 and dt == L.dt and t == L.time, (m, dt, L.dt)
+        np.testing.assert_array_equal(ens.state(m), L.state(), err_msg="instance %d" % m)
+        L.close()
+    ens.close()
+
+
 def test_fused_bit_patterns_equal_oracle():
     """Bit patterns, not just values: the batched path's state (signed zeros
     included) equals the oracle's after several steps, outflow and periodic."""

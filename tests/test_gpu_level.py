@@ -49,6 +49,12 @@ def test_config1_sedov32_outflow_bitwise():
     run_pair(cfg, cfg["steps"])
 
 
+def test_config1_sedov32_periodic_bitwise():
+    """Config 1's periodic variant: 32^3, one box, 10 steps."""
+    cfg = dict(configs.config(1), bc=0)
+    run_pair(cfg, cfg["steps"])
+
+
 def test_periodic_8boxes_bitwise():
     cfg = small_cfg(n=16, bc=0, r_init=0.1)
     run_pair(cfg, 4, boxes=configs.grid_boxes(16, 8))

@@ -154,9 +154,9 @@ __global__ __launch_bounds__(256) void K_flat(Plan p, Ctx x)
 // PPM reconstruction + characteristic tracing (Source/ppm.F90:309-743)
 __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
 {
-    int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int d = blockIdx.y;
+    int m, i, j, k, sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
     const FInst &I = p.inst[m];
     const int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const int pos = d == 0 ? i : (d == 1 ? j : k);
@@ -241,9 +241,9 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
 // (d-face index 0..n_d, transverse cells -1..n).  Species are streamed.
 __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
 {
-    int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int d = blockIdx.y;
+    int m, i, j, k, sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
     const FInst &I = p.inst[m];
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     if (pos < 0) return;
@@ -271,9 +271,9 @@ __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
 // the flux F^{t|d} of the corrected states on the t-faces of tbx[t][d].
 __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
 {
-    int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int combo = blockIdx.y;
+    int m, i, j, k, sub_;
+    if (!plan_cell_sub(p, 6, sub_, m, i, j, k)) return;
+    const int combo = sub_;
     const int t = c_pair_t[combo], d = c_pair_d[combo];
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
@@ -328,9 +328,9 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
 // on the d-faces of the valid region.
 __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 {
-    int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int d = blockIdx.y;
+    int m, i, j, k, sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
     for (int a = 0; a < 3; ++a) {
@@ -578,7 +578,12 @@ static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1)
     const int nb = plan_blocks(F, p);
     const int nt = p.bx * p.by * p.bz;
     if (nb <= 0) return;
-    dim3 g(nb, ny), b(nt);
+    // the directions / pairs of a face stage are interleaved brick by brick
+    // (plan_cell_sub): the sub-launches of a brick share inputs and run
+    // together on one XCD, so the shared reads hit its L2
+    const bool interleave = stage == ST_TRACE || stage == ST_FLUX1 || stage == ST_TRANS1 || stage == ST_FINAL;
+    if (interleave) ny = stage == ST_TRANS1 ? 6 : 3;  // the count the kernels assume
+    dim3 g(interleave ? nb * ny : nb, interleave ? 1 : ny), b(nt);
     switch (stage) {
     case ST_EST: hipLaunchKernelGGL(K_est, g, b, 0, s, p, F->ctx); break;
     case ST_PRIM: hipLaunchKernelGGL(K_prim, g, b, 0, s, p, F->ctx); break;

@@ -59,10 +59,9 @@ __device__ __forceinline__ int find_inst(const Plan &p, int b)
     return lo;
 }
 
-// Cell of this thread in the plan's range; false if outside.
-__device__ __forceinline__ bool plan_cell(const Plan &p, int &m, int &i, int &j, int &k)
+// Cell of logical block lb's thread in the plan's range; false if outside.
+__device__ __forceinline__ bool plan_cell_at(const Plan &p, int lb, int &m, int &i, int &j, int &k)
 {
-    const int lb = logical_block();
     m = find_inst(p, lb);
     const FInst &I = p.inst[m];
     const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.n[2] + p.add;
@@ -76,6 +75,27 @@ __device__ __forceinline__ bool plan_cell(const Plan &p, int &m, int &i, int &j,
     j = lj - p.shift;
     k = lk - p.shift;
     return li < e0 && lj < e1 && lk < e2;
+}
+
+__device__ __forceinline__ bool plan_cell(const Plan &p, int &m, int &i, int &j, int &k)
+{
+    return plan_cell_at(p, logical_block(), m, i, j, k);
+}
+
+// A stage with ny sub-launches (directions or pairs): either one block row
+// per sub-launch (gridDim.y == ny), or interleaved at brick granularity
+// (gridDim.y == 1, block b runs sub-launch b % ny on brick b / ny) so that
+// the sub-launches of a brick run together on one XCD and share its L2.
+__device__ __forceinline__ bool plan_cell_sub(const Plan &p, int ny, int &sub, int &m, int &i, int &j, int &k)
+{
+    int lb = logical_block();
+    if (gridDim.y > 1) {
+        sub = blockIdx.y;
+    } else {
+        sub = lb % ny;
+        lb /= ny;
+    }
+    return plan_cell_at(p, lb, m, i, j, k);
 }
 
 __device__ __forceinline__ int bmap(int i, int n, int periodic)

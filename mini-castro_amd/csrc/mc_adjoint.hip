@@ -250,8 +250,9 @@ __device__ __forceinline__ RiemannHB load_tfluxb(double *FB, const FInst &I, idx
 __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
 {
     int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int d = blockIdx.y;
+    int sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
     for (int q = 0; q < 3; ++q) {
@@ -376,8 +377,9 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
 __global__ __launch_bounds__(256) void A_trans1(Plan p, Ctx x, ACtx a)
 {
     int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int combo = blockIdx.y;
+    int sub_;
+    if (!plan_cell_sub(p, 6, sub_, m, i, j, k)) return;
+    const int combo = sub_;
     const int t = c_pair_t[combo], d = c_pair_d[combo];
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
@@ -432,8 +434,9 @@ __global__ __launch_bounds__(256) void A_trans1(Plan p, Ctx x, ACtx a)
 __global__ __launch_bounds__(256) void A_flux1(Plan p, Ctx x, ACtx a)
 {
     int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int d = blockIdx.y;
+    int sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
     const FInst &I = p.inst[m];
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     if (pos < 0) return;
@@ -453,8 +456,9 @@ __global__ __launch_bounds__(256) void A_flux1(Plan p, Ctx x, ACtx a)
 __global__ __launch_bounds__(256) void A_trace(Plan p, Ctx x, ACtx a)
 {
     int m, i, j, k;
-    if (!plan_cell(p, m, i, j, k)) return;
-    const int d = blockIdx.y;
+    int sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
     const FInst &I = p.inst[m];
     const int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const int pos = d == 0 ? i : (d == 1 ? j : k);
@@ -742,7 +746,9 @@ static void launch_adj(mc_adjoint *A, int stage, hipStream_t s)
     const int nt = p.bx * p.by * p.bz;
     if (nb <= 0) return;
     static const int ny[FST_N] = {1, 1, 1, 3, 3, 6, 3, 1};
-    dim3 g(nb, ny[stage]), b(nt);
+    // face stages: directions / pairs interleaved brick by brick, as forward
+    const bool interleave = ny[stage] > 1;
+    dim3 g(interleave ? nb * ny[stage] : nb, interleave ? 1 : ny[stage]), b(nt);
     F->ctx.tab = eos_table();
     switch (stage) {
     case FST_PRIM: hipLaunchKernelGGL(A_prim, g, b, 0, s, p, F->ctx, A->a); break;

@@ -25,29 +25,44 @@ METRIC = "zones/s at 256^3 Sedov FP64, batched vs per-box (1e-12 rel); adjoint/f
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 FP64_PEAK = 78.6e12  # MI355X FP64 vector, FLOP/s
 
-# Algorithmic HBM bytes per unit of each fused stage: every input element
-# read once and every output element written once (doubles).
-#   unit = a cell of the domain (prim, update), a cell of the one-cell ring
-#   (flat, trace per direction), a face (flux1, final per direction,
-#   trans1 per direction pair)
-STAGE_BYTES = {
-    "prim": (20 + 20 + 21) * 8,          # U in; U1, Q out
-    "flat": (4 + 2) * 8,                 # p,u,v,w in; flatn, div out
-    "trace": (21 + 1 + 2 * 19) * 8,      # Q, flatn in; minus/plus states out
-    "flux1": (2 * 19 + 2 + 20) * 8,      # states, c/gamc in; flux + Godunov out
-    "trans1": (2 * 19 + 20 + 1 + 20) * 8,  # states, d-flux, gamc in; flux out
-    "final": (2 * 19 + 2 * 20 + 1 + 20 + 2 + 22) * 8,  # states, 2 fluxes, div, U1, c/gamc in; flux out
-    "update": (3 * 22 + 20 + 20) * 8,    # 3 final fluxes, U in; U out
-}
-STAGE_UNITS_PER_LAUNCH = {  # (stage) -> function of n_cell giving units per launch
-    "prim": lambda n: n[0] * n[1] * n[2],
-    "update": lambda n: n[0] * n[1] * n[2],
-    "flat": lambda n: (n[0] + 2) * (n[1] + 2) * (n[2] + 2),
-    "trace": lambda n: 3 * (n[0] + 2) * (n[1] + 2) * (n[2] + 2),
-    "flux1": lambda n: 3 * (n[0] + 1) * (n[1] + 2) * (n[2] + 2),
-    "trans1": lambda n: 6 * (n[0] + 1) * n[1] * (n[2] + 2),
-    "final": lambda n: 3 * (n[0] + 1) * n[1] * n[2],
-}
+# Algorithmic HBM bytes per launch of each fused stage: every distinct array
+# element the launch reads counted once, every element it writes counted
+# once (doubles).  A launch covers all directions (trace, flux1, final) or
+# all six (t, d) pairs (trans1) of its stage, so an input that several
+# directions or pairs read -- the primitives Q, the traced t-states shared
+# by the two pairs of a face direction, the first-stage fluxes shared by the
+# two pairs of a flux direction -- counts once.  n: cells per side; cell
+# arrays hold n^3 elements, ring arrays (n+2)^3, face arrays (n+1)(n+2)^2
+# per direction (the [-1, n] ring in the transverse directions), the
+# transverse-stage fluxes (n+1) n (n+2) per pair and the final fluxes
+# (n+1) n^2 per direction.
+def _counts(n):
+    n0, n1, n2 = n
+    cell = n0 * n1 * n2
+    ring = (n0 + 2) * (n1 + 2) * (n2 + 2)
+    face = (n0 + 1) * (n1 + 2) * (n2 + 2)      # one direction of the face arrays (cube: all equal)
+    tface = (n0 + 1) * n1 * (n2 + 2)           # one (t, d) pair of transverse-stage faces
+    fface = (n0 + 1) * n1 * n2                 # one direction of final faces
+    return cell, ring, face, tface, fface
+
+
+def _stage_bytes(stage, n):
+    cell, ring, face, tface, fface = _counts(n)
+    NS, NF, NQ, NN = 19, 20, 21, 22   # state, flux, primitive, final-flux components
+    doubles = {
+        "prim": (20 + 20 + NQ) * cell,                             # U in; U1, Q out
+        "flat": 4 * cell + 2 * ring,                               # p, u, v, w in; flatn, div out
+        "trace": NQ * cell + ring + 3 * 2 * NS * ring,             # Q, flatn in; minus/plus states out
+        "flux1": 3 * (2 * NS + NF) * face + 2 * cell,              # states in, F^d out; c, gamc in
+        "trans1": 3 * 2 * NS * face + 3 * NF * face + cell + 6 * NF * tface,  # t-states, F^d, gamc in; F^{t|d} out
+        "final": (3 * 2 * NS * face + 6 * NF * tface + ring + 20 * cell + 2 * cell
+                  + 3 * NN * fface),                               # states, F^{t|d}, div, U1, c/gamc in; final flux out
+        "update": 3 * NN * fface + 20 * cell + 20 * cell,          # final fluxes, U in; U out
+    }[stage]
+    return 8 * doubles
+
+
+STAGES_WITH_BYTES = ("prim", "flat", "trace", "flux1", "trans1", "final", "update")
 
 
 def measured_traffic(kernel, workload):
@@ -330,10 +345,10 @@ def main():
     fused_steps_done = L.steps
     L.close()
 
-    dom = max((k for k in stages if k in STAGE_BYTES), key=lambda k: stages[k][0])
+    dom = max((k for k in stages if k in STAGES_WITH_BYTES), key=lambda k: stages[k][0])
     tot_ms, nl = stages[dom]
     avg_s = tot_ms / nl / 1e3
-    traffic_alg = STAGE_BYTES[dom] * STAGE_UNITS_PER_LAUNCH[dom](n)
+    traffic_alg = _stage_bytes(dom, n)
     achieved = traffic_alg / avg_s
     traffic = measured_traffic("mc::K_" + dom, cfg["name"])
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -355,7 +370,7 @@ def main():
     # measured rate, and the FP64 operations the fused stages issue
     # (rocprof counters, summed over the stages of one step)
     w_flop = reference_flop_per_zone_update(cfg["name"])
-    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in STAGE_BYTES]
+    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in STAGES_WITH_BYTES]
     hw_step = sum(hw) if all(hw) else None
     roofline["step"] = {
         "unit_work": "zone-update",

@@ -5,7 +5,7 @@
 //
 //   K_prim    cells   clean_state(S_old), clean_state(Sborder), ctoprim
 //   K_flat    cells+1 shock flattening coefficient, node-centred div(u)
-//   K_trace   cells+1 PPM edge states in x, y, z (blockIdx.y = direction)
+//   K_trace   cells+1 PPM edge states in x, y, z
 //   K_flux1   faces   CGF fluxes of the PPM states           F^d
 //   K_trans1  faces   one-direction transverse update + flux  F^{t|d}
 //   K_final   faces   two-direction transverse update + final flux,
@@ -26,6 +26,10 @@
 // Quantities the reference evaluates on ghost cells of the domain (traces
 // and fluxes up to one cell outside) are evaluated on a one-cell ring
 // around each instance ("extended" arrays).
+//
+// The directions (trace, flux1, final) or (t, d) pairs (trans1) of a face
+// stage are sub-launches of one launch, interleaved brick by brick
+// (plan_cell_sub), so the sub-launches of a brick share its XCD's L2.
 //
 // Layout: per instance, structure-of-arrays; a cell array with C components
 // holds component c of cell (i,j,k) at C*off + c*ncell + i + n0*(j + n1*k);

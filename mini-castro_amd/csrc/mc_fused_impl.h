@@ -111,8 +111,9 @@ __device__ __forceinline__ int bmap(int i, int n, int periodic)
 typedef unsigned idx_t;
 
 // Every array base CA / EA index is uniform across a wavefront: a workgroup
-// belongs to one instance, and its direction / pair index comes from
-// blockIdx.y or compile-time loops (a component chosen per lane goes through
+// belongs to one instance, and its direction / pair index comes from the
+// block index (plan_cell_sub) or compile-time loops (a component chosen per
+// lane goes through
 // CA_lane instead).  Stating it keeps the base in scalar registers (one 32-bit
 // vector offset per access instead of a 64-bit address).
 __device__ __forceinline__ double *uniform_ptr(double *p)

@@ -113,9 +113,9 @@ typedef unsigned idx_t;
 // Every array base CA / EA index is uniform across a wavefront: a workgroup
 // belongs to one instance, and its direction / pair index comes from the
 // block index (plan_cell_sub) or compile-time loops (a component chosen per
-// lane goes through
-// CA_lane instead).  Stating it keeps the base in scalar registers (one 32-bit
-// vector offset per access instead of a 64-bit address).
+// lane goes through CA_lane instead).  Stating it keeps the base in scalar
+// registers (one 32-bit vector offset per access instead of a 64-bit
+// address).
 __device__ __forceinline__ double *uniform_ptr(double *p)
 {
     const unsigned long long v = reinterpret_cast<unsigned long long>(p);

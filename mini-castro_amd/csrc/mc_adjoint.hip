@@ -790,7 +790,7 @@ mc_adjoint_t *mc_adjoint_create(const int *n_cell, const double *prob_lo, const 
         return nullptr;
     }
     A->nD = (long)n_cell[0] * n_cell[1] * n_cell[2];
-    A->nE = (long)(n_cell[0] + 2) * (n_cell[1] + 2) * (n_cell[2] + 2);
+    A->nE = fused_next(A->F, 0);  // the reverse kernels index with the engine's stride
     A->max_steps = max_steps;
     // intermediates: U1b, Qb (cells), FLb, DVb, SMb, SPb, FDb, FTb, FNb (extended)
     const long nD = A->nD, nE = A->nE;

@@ -26,6 +26,7 @@ int fused_import(FusedLevel *F, const double *arena, hipStream_t s);
 int fused_export(FusedLevel *F, double *arena, hipStream_t s);
 double *fused_state(FusedLevel *F, int m);  // instance m state (20 comps, cell layout)
 long fused_ncell(FusedLevel *F, int m);
+long fused_next(const FusedLevel *F, int m);  // component stride of instance m's extended arrays
 int fused_ninst(FusedLevel *F);
 int fused_estimate(FusedLevel *F, hipStream_t s);
 int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool use_graph);

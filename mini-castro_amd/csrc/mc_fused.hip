@@ -208,16 +208,9 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     }
     const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
     double *SP = x.SP[d], *SM = x.SM[d];
-    for (int ns = 0; ns < NSPEC; ++ns) {
-        double s[5], sm, sp, ipv, imv;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, FQ_X + ns, cs[q]);
-        ppm_reconstruct(s, flatn, sm, sp);
-        const double s6 = ppm_s6(sm, sp, s[2]);
-        ppm_int_wave(sm, sp, s6, un, dtdx, ipv, imv);
-        if (do_p) EA(SP, I, NFS, FS_X + ns, ep) = imv;
-        if (do_m) EA(SM, I, NFS, FS_X + ns, em) = ipv;
-    }
+    const SpeciesPairs XP = species_pairs(SP, I, NFS, FS_X), XM = species_pairs(SM, I, NFS, FS_X);
+    double *const QX = x.Q + ((long)NFQ * I.offD + (long)FQ_X * I.nD);  // species of Q, taken before the stores
+    const long nD = I.nD;
     const int sun = FS_U + d, sut = d == 0 ? FS_V : (d == 1 ? FS_W : FS_U), sutt = d == 0 ? FS_W : (d == 1 ? FS_U : FS_V);
     if (do_p) {
         double r, u_, e, pp;
@@ -238,6 +231,23 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
         EA(SM, I, NFS, FS_P, em) = pp;
         EA(SM, I, NFS, sut, em) = Ip_ut;
         EA(SM, I, NFS, sutt, em) = Ip_utt;
+    }
+    // species last: their stores come after every read of the instance table
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        double ipv[2] = {0.0, 0.0}, imv[2] = {0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ns = 2 * pr + h;
+            if (ns >= NSPEC) break;
+            double s[5], sm, sp;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) s[q] = at(QX + ns * nD, cs[q]);
+            ppm_reconstruct(s, flatn, sm, sp);
+            const double s6 = ppm_s6(sm, sp, s[2]);
+            ppm_int_wave(sm, sp, s6, un, dtdx, ipv[h], imv[h]);
+        }
+        if (do_p) SP2(XP, pr, ep) = dbl2{imv[0], imv[1]};
+        if (do_m) SP2(XM, pr, em) = dbl2{ipv[0], ipv[1]};
     }
 }
 
@@ -262,12 +272,15 @@ __global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
     // a side whose weight is zero contributes +-0 to fpl*xl + fmr*xr: its
     // species are not read
     const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
+    const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
+                       XF = species_pairs(F, I, NFF, FF_X);
 #pragma unroll 1
-    for (int n = 0; n < NSPEC; ++n) {
-        const double xl = useL ? EA(x.SM[d], I, NFS, FS_X + n, e) : 0.0;
-        const double xr = useR ? EA(x.SP[d], I, NFS, FS_X + n, e) : 0.0;
-        const double gx = o.fpl * xl + o.fmr * xr;
-        EA(F, I, NFF, FF_X + n, e) = o.f_rho * gx;
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const dbl2 xl = useL ? (dbl2)SP2(XL, pr, e) : dbl2{0.0, 0.0};
+        const dbl2 xr = useR ? (dbl2)SP2(XR, pr, e) : dbl2{0.0, 0.0};
+        const double gx0 = o.fpl * xl.x + o.fmr * xr.x;
+        const double gx1 = o.fpl * xl.y + o.fmr * xr.y;
+        SP2(XF, pr, e) = dbl2{o.f_rho * gx0, o.f_rho * gx1};
     }
 }
 
@@ -311,19 +324,27 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
     face_riemann_h(x, I, t, i, j, k, side[0], side[1], o);
     double *F = x.FT[combo], *FDd = x.FD[d];
     store_tflux_h(F, I, e, t, o);
-    // only the side(s) with a nonzero Riemann weight are corrected and read
+    // only the side(s) with a nonzero Riemann weight are corrected and read;
+    // species go in pairs (SP2)
     const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
+    const SpeciesPairs XL = species_pairs(x.SM[t], I, NFS, FS_X), XR = species_pairs(x.SP[t], I, NFS, FS_X),
+                       XD = species_pairs(FDd, I, NFF, FF_X), XF = species_pairs(F, I, NFF, FF_X);
 #pragma unroll
-    for (int n = 0; n < NSPEC; ++n) {
-        const double x0 = useL ? trans1_species(rho[0], EA(x.SM[t], I, NFS, FS_X + n, e),
-                                                EA(FDd, I, NFF, FF_X + n, er[0]), EA(FDd, I, NFF, FF_X + n, el[0]),
-                                                cdt, rrnew[0])
-                               : 0.0;
-        const double x1 = useR ? trans1_species(rho[1], EA(x.SP[t], I, NFS, FS_X + n, e),
-                                                EA(FDd, I, NFF, FF_X + n, er[1]), EA(FDd, I, NFF, FF_X + n, el[1]),
-                                                cdt, rrnew[1])
-                               : 0.0;
-        EA(F, I, NFF, FF_X + n, e) = o.f_rho * (o.fpl * x0 + o.fmr * x1);
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const bool two = 2 * pr + 1 < NSPEC;  // the last pair holds one species
+        double x0[2] = {0.0, 0.0}, x1[2] = {0.0, 0.0};
+        if (useL) {
+            const dbl2 s = SP2(XL, pr, e), fr = SP2(XD, pr, er[0]), fl = SP2(XD, pr, el[0]);
+            x0[0] = trans1_species(rho[0], s.x, fr.x, fl.x, cdt, rrnew[0]);
+            if (two) x0[1] = trans1_species(rho[0], s.y, fr.y, fl.y, cdt, rrnew[0]);
+        }
+        if (useR) {
+            const dbl2 s = SP2(XR, pr, e), fr = SP2(XD, pr, er[1]), fl = SP2(XD, pr, el[1]);
+            x1[0] = trans1_species(rho[1], s.x, fr.x, fl.x, cdt, rrnew[1]);
+            if (two) x1[1] = trans1_species(rho[1], s.y, fr.y, fl.y, cdt, rrnew[1]);
+        }
+        SP2(XF, pr, e) = dbl2{o.f_rho * (o.fpl * x0[0] + o.fmr * x1[0]),
+                                            two ? o.f_rho * (o.fpl * x0[1] + o.fmr * x1[1]) : 0.0};
     }
 }
 
@@ -398,20 +419,33 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     EA(FN, I, NFN, UTEMP, e) = 0.0;
     double fs[NSPEC], sum = 0.0;
     const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
+    const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
+                       X2 = species_pairs(F2, I, NFF, FF_X), X3 = species_pairs(F3, I, NFF, FF_X);
 #pragma unroll
-    for (int n = 0; n < NSPEC; ++n) {
-        const double x0 = useL ? trans2_species(rho[0], EA(x.SM[d], I, NFS, FS_X + n, e),
-                                                EA(F2, I, NFF, FF_X + n, r2[0]), EA(F2, I, NFF, FF_X + n, l2[0]),
-                                                EA(F3, I, NFF, FF_X + n, r3[0]), EA(F3, I, NFF, FF_X + n, l2[0]),
-                                                h2, h3, rrnew[0])
-                               : 0.0;
-        const double x1 = useR ? trans2_species(rho[1], EA(x.SP[d], I, NFS, FS_X + n, e),
-                                                EA(F2, I, NFF, FF_X + n, r2[1]), EA(F2, I, NFF, FF_X + n, l2[1]),
-                                                EA(F3, I, NFF, FF_X + n, r3[1]), EA(F3, I, NFF, FF_X + n, l2[1]),
-                                                h2, h3, rrnew[1])
-                               : 0.0;
-        fs[n] = AVN(UFS + n, o.f_rho * (o.fpl * x0 + o.fmr * x1));
-        sum = sum + fs[n];
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const bool two = 2 * pr + 1 < NSPEC;  // the last pair holds one species
+        double x0[2] = {0.0, 0.0}, x1[2] = {0.0, 0.0};
+        if (useL) {
+            const dbl2 s = SP2(XL, pr, e);
+            const dbl2 a = SP2(X2, pr, r2[0]), b = SP2(X2, pr, l2[0]);
+            const dbl2 c3 = SP2(X3, pr, r3[0]), d3 = SP2(X3, pr, l2[0]);
+            x0[0] = trans2_species(rho[0], s.x, a.x, b.x, c3.x, d3.x, h2, h3, rrnew[0]);
+            if (two) x0[1] = trans2_species(rho[0], s.y, a.y, b.y, c3.y, d3.y, h2, h3, rrnew[0]);
+        }
+        if (useR) {
+            const dbl2 s = SP2(XR, pr, e);
+            const dbl2 a = SP2(X2, pr, r2[1]), b = SP2(X2, pr, l2[1]);
+            const dbl2 c3 = SP2(X3, pr, r3[1]), d3 = SP2(X3, pr, l2[1]);
+            x1[0] = trans2_species(rho[1], s.x, a.x, b.x, c3.x, d3.x, h2, h3, rrnew[1]);
+            if (two) x1[1] = trans2_species(rho[1], s.y, a.y, b.y, c3.y, d3.y, h2, h3, rrnew[1]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int n = 2 * pr + h;
+            if (n >= NSPEC) break;
+            fs[n] = AVN(UFS + n, o.f_rho * (o.fpl * x0[h] + o.fmr * x1[h]));
+            sum = sum + fs[n];
+        }
     }
 #undef AVN
     const double fac = (sum != 0.0) ? frho / sum : 1.0;
@@ -620,6 +654,7 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
         I.periodic = sp.bc == 0;
         I.nD = (long)I.n[0] * I.n[1] * I.n[2];
         I.nE = (long)I.ext[0] * I.ext[1] * I.ext[2];
+        I.nE += I.nE & 1;  // even: the species pairs (SP2) stay 16-byte aligned
         if (I.nE >= (1L << 29)) {  // per-instance byte offsets are 32-bit
             set_error("fused engine: an instance may hold at most 2^29 extended cells");
             delete F;
@@ -841,6 +876,7 @@ void fused_set_intermediates(FusedLevel *F, const FusedArrays &S)
 
 double *fused_state(FusedLevel *F, int m) { return F->ctx.U + (long)NVAR * F->hinst[m].offD; }
 long fused_ncell(FusedLevel *F, int m) { return F->hinst[m].nD; }
+long fused_next(const FusedLevel *F, int m) { return F->hinst[m].nE; }
 int fused_ninst(FusedLevel *F) { return F->ninst; }
 
 // estdt of the current state of every instance into dt_est (no cleaning)

@@ -13,10 +13,13 @@ namespace mc {
 
 // primitive components of the fused path
 enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_X, FQ_C = FQ_X + NSPEC, FQ_G, NFQ };
+// The passive species of the state and flux arrays are stored in pairs
+// (SP2 below): 2 * NSPEC_PAIRS component slots for NSPEC species.
+constexpr int NSPEC_PAIRS = (NSPEC + 1) / 2;
 // traced / corrected interface states (velocities in absolute order)
-enum { FS_RHO = 0, FS_U, FS_V, FS_W, FS_P, FS_RE, FS_X, NFS = FS_X + NSPEC };
+enum { FS_RHO = 0, FS_U, FS_V, FS_W, FS_P, FS_RE, FS_X, NFS = FS_X + 2 * NSPEC_PAIRS };
 // transverse-stage fluxes: conserved URHO..UEDEN, species, Godunov p and normal u
-enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_X, FF_GP = FF_X + NSPEC, FF_GU, NFF };
+enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_X, FF_GP = FF_X + 2 * NSPEC_PAIRS, FF_GU, NFF };
 // final fluxes: NVAR conserved components, Godunov p and normal u
 enum { FN_GP = NVAR, FN_GU, NFN };
 
@@ -133,6 +136,34 @@ __device__ __forceinline__ gdouble &at(double *base, idx_t idx)
     typedef __attribute__((address_space(1))) char gchar;
     gchar *g = (gchar *)uniform_ptr(base);
     return *(gdouble *)(g + idx * 8u);
+}
+
+// The species of a state or flux array (C components per element, species
+// block at component cX) in pairs: species 2p and 2p+1 share components
+// cX+2p and cX+2p+1 as one interleaved region, element idx of species n at
+// double offset (cX + 2*(n/2))*nE + 2*idx + n%2 of the instance's part.  One
+// 16-byte access per lane moves both species of a pair, half the load and
+// store instructions of separate components.  nE and the instance offsets
+// are even, so the regions are 16-byte aligned.  The base and pair stride
+// are taken once, before a kernel's species loop, so that the loop's stores
+// (which may alias the instance table as far as the compiler knows) do not
+// force the instance fields to be loaded again in every pass.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) dbl2 gdbl2;
+struct SpeciesPairs {
+    double *base;
+    long stride;  // doubles from one pair region to the next
+};
+__device__ __forceinline__ SpeciesPairs species_pairs(double *a, const FInst &I, int C, int cX)
+{
+    return SpeciesPairs{a + ((long)C * I.offE + (long)cX * I.nE), 2 * I.nE};
+}
+// pair p at element idx; the lane offset is 64-bit (it can pass 2^32 bytes)
+__device__ __forceinline__ gdbl2 &SP2(const SpeciesPairs &S, int p, idx_t idx)
+{
+    typedef __attribute__((address_space(1))) char gchar;
+    gchar *g = (gchar *)uniform_ptr(S.base + p * S.stride);
+    return *(gdbl2 *)(g + (unsigned long long)idx * 16u);
 }
 
 // index of a (boundary-mapped) cell in the cell space of instance I

@@ -105,11 +105,20 @@ __global__ __launch_bounds__(256) void A_update(Plan p, Ctx x, ACtx a)
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) ua[n] = u0[n];
     clean_hydro(ua, x.tab);
+    const SpeciesPairs P0 = species_pairs(F0, I, NFN, FN_X), P1 = species_pairs(F1, I, NFN, FN_X),
+                       P2 = species_pairs(F2, I, NFN, FN_X);
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) {
-        double s = 0.0 + (EA(F0, I, NFN, n, e) * a0 - EA(F0, I, NFN, n, ex) * a0 + EA(F1, I, NFN, n, e) * a1 -
-                          EA(F1, I, NFN, n, ey) * a1 + EA(F2, I, NFN, n, e) * a2 - EA(F2, I, NFN, n, ez) * a2) *
-                             volinv;
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const dbl2 f0 = SP2(P0, pr, e), f0x = SP2(P0, pr, ex), f1 = SP2(P1, pr, e), f1y = SP2(P1, pr, ey),
+                   f2 = SP2(P2, pr, e), f2z = SP2(P2, pr, ez);
+        const int n = FN_X + 2 * pr;
+        u1[n] = ua[n] + dt * flux_div(f0.x, f0x.x, f1.x, f1y.x, f2.x, f2z.x, a0, a1, a2, volinv);
+        if (n + 1 < NVAR) u1[n + 1] = ua[n + 1] + dt * flux_div(f0.y, f0x.y, f1.y, f1y.y, f2.y, f2z.y, a0, a1, a2, volinv);
+    }
+#pragma unroll
+    for (int n = 0; n < FN_X; ++n) {
+        double s = flux_div(EA(F0, I, NFN, n, e), EA(F0, I, NFN, n, ex), EA(F1, I, NFN, n, e), EA(F1, I, NFN, n, ey),
+                            EA(F2, I, NFN, n, e), EA(F2, I, NFN, n, ez), a0, a1, a2, volinv);
         if (n == UEINT) {
             const double pdivu = HALF * (EA(F0, I, NFN, FN_GP, ex) + EA(F0, I, NFN, FN_GP, e)) *
                                      (EA(F0, I, NFN, FN_GU, ex) - EA(F0, I, NFN, FN_GU, e)) / dx0 +

@@ -409,6 +409,9 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 #define AVN(n, fval) ((fval) + dxd * (div1 * (CA(x.U1, I, NVAR, n, cr) - CA(x.U1, I, NVAR, n, cl))))
     const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
     double *FN = x.FN[d];
+    const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
+                       X2 = species_pairs(F2, I, NFF, FF_X), X3 = species_pairs(F3, I, NFF, FF_X),
+                       XN = species_pairs(FN, I, NFN, FN_X);
     const double frho = AVN(URHO, o.f_rho);
     EA(FN, I, NFN, URHO, e) = frho;
     EA(FN, I, NFN, mn, e) = AVN(mn, o.f_mn);
@@ -419,8 +422,6 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     EA(FN, I, NFN, UTEMP, e) = 0.0;
     double fs[NSPEC], sum = 0.0;
     const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
-    const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
-                       X2 = species_pairs(F2, I, NFF, FF_X), X3 = species_pairs(F3, I, NFF, FF_X);
 #pragma unroll
     for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
         const bool two = 2 * pr + 1 < NSPEC;  // the last pair holds one species
@@ -450,7 +451,8 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 #undef AVN
     const double fac = (sum != 0.0) ? frho / sum : 1.0;
 #pragma unroll
-    for (int n = 0; n < NSPEC; ++n) EA(FN, I, NFN, UFS + n, e) = fs[n] * fac;
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr)
+        SP2(XN, pr, e) = dbl2{fs[2 * pr] * fac, 2 * pr + 1 < NSPEC ? fs[2 * pr + 1] * fac : 0.0};
     EA(FN, I, NFN, FN_GP, e) = o.g_p;
     EA(FN, I, NFN, FN_GU, e) = o.g_un;
 }
@@ -475,16 +477,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
         const idx_t e = eidx(I, i, j, k);
         const idx_t ex = eidx(I, i + 1, j, k), ey = eidx(I, i, j + 1, k), ez = eidx(I, i, j, k + 1);
         double *F0 = x.FN[0], *F1 = x.FN[1], *F2 = x.FN[2];
+        const SpeciesPairs P0 = species_pairs(F0, I, NFN, FN_X), P1 = species_pairs(F1, I, NFN, FN_X),
+                           P2 = species_pairs(F2, I, NFN, FN_X);
         const double dt = x.dt_cur[m];
         double u[NVAR];
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
         clean_cell(u, x.tab); /* S_old clean, Castro_advance.cpp:51 */
 #pragma unroll
-        for (int n = 0; n < NVAR; ++n) {
-            double s = 0.0 + (EA(F0, I, NFN, n, e) * a0 - EA(F0, I, NFN, n, ex) * a0 + EA(F1, I, NFN, n, e) * a1 -
-                              EA(F1, I, NFN, n, ey) * a1 + EA(F2, I, NFN, n, e) * a2 - EA(F2, I, NFN, n, ez) * a2) *
-                                 volinv;
+        for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+            const dbl2 f0 = SP2(P0, pr, e), f0x = SP2(P0, pr, ex), f1 = SP2(P1, pr, e), f1y = SP2(P1, pr, ey),
+                       f2 = SP2(P2, pr, e), f2z = SP2(P2, pr, ez);
+            const int n = FN_X + 2 * pr;
+            u[n] = u[n] + dt * flux_div(f0.x, f0x.x, f1.x, f1y.x, f2.x, f2z.x, a0, a1, a2, volinv);
+            if (n + 1 < NVAR) u[n + 1] = u[n + 1] + dt * flux_div(f0.y, f0x.y, f1.y, f1y.y, f2.y, f2z.y, a0, a1, a2, volinv);
+        }
+#pragma unroll
+        for (int n = 0; n < FN_X; ++n) {
+            double s = flux_div(EA(F0, I, NFN, n, e), EA(F0, I, NFN, n, ex), EA(F1, I, NFN, n, e),
+                                EA(F1, I, NFN, n, ey), EA(F2, I, NFN, n, e), EA(F2, I, NFN, n, ez), a0, a1, a2, volinv);
             if (n == UEINT) {
                 const double pdivu =
                     HALF * (EA(F0, I, NFN, FN_GP, ex) + EA(F0, I, NFN, FN_GP, e)) *

@@ -20,8 +20,9 @@ constexpr int NSPEC_PAIRS = (NSPEC + 1) / 2;
 enum { FS_RHO = 0, FS_U, FS_V, FS_W, FS_P, FS_RE, FS_X, NFS = FS_X + 2 * NSPEC_PAIRS };
 // transverse-stage fluxes: conserved URHO..UEDEN, species, Godunov p and normal u
 enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_X, FF_GP = FF_X + 2 * NSPEC_PAIRS, FF_GU, NFF };
-// final fluxes: NVAR conserved components, Godunov p and normal u
-enum { FN_GP = NVAR, FN_GU, NFN };
+// final fluxes: NVAR conserved components (species in pairs), Godunov p and normal u
+static_assert(UFS + NSPEC == NVAR, "the species are the last conserved components");
+enum { FN_X = UFS, FN_GP = FN_X + 2 * NSPEC_PAIRS, FN_GU, NFN };
 
 struct FInst {
     int n[3];
@@ -164,6 +165,14 @@ __device__ __forceinline__ gdbl2 &SP2(const SpeciesPairs &S, int p, idx_t idx)
     typedef __attribute__((address_space(1))) char gchar;
     gchar *g = (gchar *)uniform_ptr(S.base + p * S.stride);
     return *(gdbl2 *)(g + (unsigned long long)idx * 16u);
+}
+
+// Flux divergence of one conserved component (Source/Castro_hydro_nd.F90:
+// 382-391): the six face fluxes times their areas, over the cell volume.
+__device__ __forceinline__ double flux_div(double f0, double f0x, double f1, double f1y, double f2, double f2z,
+                                           double a0, double a1, double a2, double volinv)
+{
+    return 0.0 + (f0 * a0 - f0x * a0 + f1 * a1 - f1y * a1 + f2 * a2 - f2z * a2) * volinv;
 }
 
 // index of a (boundary-mapped) cell in the cell space of instance I

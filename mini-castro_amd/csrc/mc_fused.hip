@@ -92,12 +92,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     {
         double xs[NSPEC];
         ctoprim_pre(u, a, xs);
+        const SpeciesPairs XQ = q_species(x.QX, I);
         CA(x.Q, I, NFQ, FQ_RHO, c) = a.rho;
         CA(x.Q, I, NFQ, FQ_U, c) = a.u;
         CA(x.Q, I, NFQ, FQ_V, c) = a.v;
         CA(x.Q, I, NFQ, FQ_W, c) = a.w;
 #pragma unroll
-        for (int n = 0; n < NSPEC; ++n) CA(x.Q, I, NFQ, FQ_X + n, c) = xs[n];
+        for (int pr = 0; pr < NSPEC_PAIRS; ++pr)
+            SP2(XQ, pr, c) = dbl2{xs[2 * pr], 2 * pr + 1 < NSPEC ? xs[2 * pr + 1] : 0.0};
     }
     Prim q;
     ctoprim_eos(x.tab, a, q);
@@ -209,8 +211,7 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
     double *SP = x.SP[d], *SM = x.SM[d];
     const SpeciesPairs XP = species_pairs(SP, I, NFS, FS_X), XM = species_pairs(SM, I, NFS, FS_X);
-    double *const QX = x.Q + ((long)NFQ * I.offD + (long)FQ_X * I.nD);  // species of Q, taken before the stores
-    const long nD = I.nD;
+    const SpeciesPairs XQ = q_species(x.QX, I);
     const int sun = FS_U + d, sut = d == 0 ? FS_V : (d == 1 ? FS_W : FS_U), sutt = d == 0 ? FS_W : (d == 1 ? FS_U : FS_V);
     if (do_p) {
         double r, u_, e, pp;
@@ -235,13 +236,16 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     // species last: their stores come after every read of the instance table
     for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
         double ipv[2] = {0.0, 0.0}, imv[2] = {0.0, 0.0};
+        dbl2 s2[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s2[q] = SP2(XQ, pr, cs[q]);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int ns = 2 * pr + h;
             if (ns >= NSPEC) break;
             double s[5], sm, sp;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) s[q] = at(QX + ns * nD, cs[q]);
+            for (int q = 0; q < 5; ++q) s[q] = h == 0 ? s2[q].x : s2[q].y;
             ppm_reconstruct(s, flatn, sm, sp);
             const double s6 = ppm_s6(sm, sp, s[2]);
             ppm_int_wave(sm, sp, s6, un, dtdx, ipv[h], imv[h]);
@@ -654,7 +658,7 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
 {
     FusedLevel *F = new FusedLevel();
     F->ninst = (int)spec.size();
-    long offD = 0, offE = 0;
+    long offD = 0, offE = 0, offQ = 0;
     for (const auto &sp : spec) {
         FInst I;
         for (int d = 0; d < 3; ++d) {
@@ -671,14 +675,18 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
             delete F;
             return nullptr;
         }
+        I.nQ = I.nD + (I.nD & 1);  // even, like nE
         I.offD = offD;
         I.offE = offE;
+        I.offQ = offQ;
         offD += I.nD;
         offE += I.nE;
+        offQ += I.nQ;
         F->hinst.push_back(I);
     }
     F->totD = offD;
     F->totE = offE;
+    F->totQ = offQ;
     for (int s = 0; s < NSTAGE; ++s) F->tiling[s] = KernelTiling{64, 2, 2};
     F->tiling[ST_EST] = F->tiling[ST_PRIM] = F->tiling[ST_UPDATE] = KernelTiling{64, 4, 1};
     bool ok = !check_hip(hipMalloc(&F->inst, sizeof(FInst) * F->ninst), "hipMalloc(inst)") &&
@@ -692,6 +700,7 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
     A(x.U, NVAR * offD);
     A(x.U1, NVAR * offD);
     A(x.Q, NFQ * offD);
+    A(x.QX, 2L * NSPEC_PAIRS * offQ);
     A(x.FL, offE);
     A(x.DV, offE);
     for (int d = 0; d < 3; ++d) {
@@ -765,7 +774,7 @@ void fused_destroy(FusedLevel *F)
     if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
     if (F->graph) (void)hipGraphDestroy(F->graph);
     Ctx &x = F->ctx;
-    double *ps[] = {x.U, x.U1, x.Q, x.FL, x.DV, x.dt_cur, x.dt_est, x.time};
+    double *ps[] = {x.U, x.U1, x.Q, x.QX, x.FL, x.DV, x.dt_cur, x.dt_est, x.time};
     for (double *p : ps)
         if (p) (void)hipFree(p);
     for (int d = 0; d < 3; ++d) {
@@ -802,10 +811,11 @@ int fused_export(FusedLevel *F, double *arena, hipStream_t s)
 
 // ---- intermediate sets
 template <class Fn>
-static void for_each_intermediate(FusedArrays &S, long totD, long totE, Fn f)
+static void for_each_intermediate(FusedArrays &S, long totD, long totE, long totQ, Fn f)
 {
     f(S.U1, NVAR * totD);
     f(S.Q, NFQ * totD);
+    f(S.QX, 2L * NSPEC_PAIRS * totQ);
     f(S.FL, totE);
     f(S.DV, totE);
     for (int d = 0; d < 3; ++d) {
@@ -821,7 +831,7 @@ size_t fused_intermediate_bytes(const FusedLevel *F)
 {
     FusedArrays S{};
     size_t n = 0;
-    for_each_intermediate(S, F->totD, F->totE, [&](double *&, long c) { n += sizeof(double) * std::max(c, 1L); });
+    for_each_intermediate(S, F->totD, F->totE, F->totQ, [&](double *&, long c) { n += sizeof(double) * std::max(c, 1L); });
     return n;
 }
 
@@ -829,7 +839,7 @@ int fused_alloc_intermediates(const FusedLevel *F, FusedArrays &S)
 {
     S = FusedArrays{};
     bool ok = true;
-    for_each_intermediate(S, F->totD, F->totE, [&](double *&p, long c) {
+    for_each_intermediate(S, F->totD, F->totE, F->totQ, [&](double *&p, long c) {
         if (ok && check_hip(hipMalloc(&p, sizeof(double) * std::max(c, 1L)), "hipMalloc(intermediates)")) ok = false;
     });
     if (!ok) fused_free_intermediates(S);
@@ -839,7 +849,7 @@ int fused_alloc_intermediates(const FusedLevel *F, FusedArrays &S)
 void fused_free_intermediates(FusedArrays &S)
 {
     FusedArrays T = S;
-    for_each_intermediate(T, 0, 0, [&](double *&p, long) {
+    for_each_intermediate(T, 0, 0, 0, [&](double *&p, long) {
         if (p) (void)hipFree(p);
         p = nullptr;
     });
@@ -852,6 +862,7 @@ FusedArrays fused_get_intermediates(const FusedLevel *F)
     FusedArrays S;
     S.U1 = x.U1;
     S.Q = x.Q;
+    S.QX = x.QX;
     S.FL = x.FL;
     S.DV = x.DV;
     for (int d = 0; d < 3; ++d) {
@@ -869,6 +880,7 @@ void fused_set_intermediates(FusedLevel *F, const FusedArrays &S)
     Ctx &x = F->ctx;
     x.U1 = S.U1;
     x.Q = S.Q;
+    x.QX = S.QX;
     x.FL = S.FL;
     x.DV = S.DV;
     for (int d = 0; d < 3; ++d) {

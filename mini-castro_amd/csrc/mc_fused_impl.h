@@ -11,8 +11,9 @@
 
 namespace mc {
 
-// primitive components of the fused path
-enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_X, FQ_C = FQ_X + NSPEC, FQ_G, NFQ };
+// primitive components of the fused path; the species of q live in their
+// own array of pairs (Ctx::QX)
+enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_C, FQ_G, NFQ };
 // The passive species of the state and flux arrays are stored in pairs
 // (SP2 below): 2 * NSPEC_PAIRS component slots for NSPEC species.
 constexpr int NSPEC_PAIRS = (NSPEC + 1) / 2;
@@ -30,6 +31,7 @@ struct FInst {
     double dx[3];
     long offD, offE;  // cell offsets in the cell / extended index spaces
     long nD, nE;      // cell counts
+    long offQ, nQ;    // species pairs of q: offset and cell count, both even
     int ext[3];       // n + 2
 };
 
@@ -45,7 +47,7 @@ struct Plan {
 
 struct Ctx {
     const FInst *inst;
-    double *U, *U1, *Q, *FL, *DV;
+    double *U, *U1, *Q, *QX, *FL, *DV;
     double *SM[3], *SP[3], *FD[3], *FT[6], *FN[3];
     double *dt_cur, *dt_est, *time;
     int *steps;
@@ -158,6 +160,11 @@ struct SpeciesPairs {
 __device__ __forceinline__ SpeciesPairs species_pairs(double *a, const FInst &I, int C, int cX)
 {
     return SpeciesPairs{a + ((long)C * I.offE + (long)cX * I.nE), 2 * I.nE};
+}
+// the species of the primitives q, over the instance's cells
+__device__ __forceinline__ SpeciesPairs q_species(double *QX, const FInst &I)
+{
+    return SpeciesPairs{QX + 2L * NSPEC_PAIRS * I.offQ, 2 * I.nQ};
 }
 // pair p at element idx; the lane offset is 64-bit (it can pass 2^32 bytes)
 __device__ __forceinline__ gdbl2 &SP2(const SpeciesPairs &S, int p, idx_t idx)
@@ -293,7 +300,7 @@ struct FusedLevel {
     std::vector<FInst> hinst;
     int ninst = 0;
     FInst *inst = nullptr;
-    long totD = 0, totE = 0;
+    long totD = 0, totE = 0, totQ = 0;
     Ctx ctx;
     double *arena_buf = nullptr;
     // plans: one block table per (tiling, ghost) pair

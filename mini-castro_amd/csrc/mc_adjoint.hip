@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void A_final(Plan p, Ctx x, ACtx a)
 #pragma unroll
     for (int h = 0; h < NHB; ++h) {
         const int n = hb_comp(h);
-        const double ur = CA(x.U1, I, NVAR, n, cr), ul = CA(x.U1, I, NVAR, n, cl);
+        const double ur = CA(x.U1, I, NU1, n, cr), ul = CA(x.U1, I, NU1, n, cl);
         div1_b += fnb[h] * dxd * (ur - ul);
         const double g = fnb[h] * dxd * div1;
         aadd(CAp(a.U1b, I, NHB, h, cr), g);
@@ -663,7 +663,14 @@ __global__ __launch_bounds__(256) void A_prim(Plan p, Ctx x, ACtx a)
     for (int n = 0; n < NVAR; ++n) ua[n] = u0[n];
     clean_hydro(ua, x.tab);
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) u1[n] = CA(x.U1, I, NVAR, n, c);  // clean(clean(U)) of the forward
+    for (int n = 0; n < NU1; ++n) u1[n] = CA(x.U1, I, NU1, n, c);  // clean(clean(U)) of the forward
+    const SpeciesPairs XU = cell_species(x.U1X, I);
+#pragma unroll
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const dbl2 v = SP2(XU, pr, c);
+        u1[UFS + 2 * pr] = v.x;
+        if (2 * pr + 1 < NSPEC) u1[UFS + 2 * pr + 1] = v.y;
+    }
     double qb[NQB], ub[NHB];
 #pragma unroll
     for (int q = 0; q < NQB; ++q) qb[q] = CA(a.Qb, I, NQB, q, c);

@@ -45,7 +45,7 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
 // traces, fluxes).  A caller may give the engine another set to write into,
 // to keep a step's intermediates instead of recomputing them later.
 struct FusedArrays {
-    double *U1, *Q, *QX, *FL, *DV;
+    double *U1, *U1X, *Q, *QX, *FL, *DV;
     double *SM[3], *SP[3], *FD[3], *FT[6], *FN[3];
 };
 size_t fused_intermediate_bytes(const FusedLevel *F);

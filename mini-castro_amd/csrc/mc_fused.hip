@@ -85,14 +85,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     clean_cell(u, x.tab);
     clean_cell(u, x.tab);
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) CA(x.U1, I, NVAR, n, c) = u[n];
+    for (int n = 0; n < NU1; ++n) CA(x.U1, I, NU1, n, c) = u[n];
+    {
+        const SpeciesPairs XU = cell_species(x.U1X, I);
+#pragma unroll
+        for (int pr = 0; pr < NSPEC_PAIRS; ++pr)
+            SP2(XU, pr, c) = dbl2{u[UFS + 2 * pr], 2 * pr + 1 < NSPEC ? u[UFS + 2 * pr + 1] : 0.0};
+    }
     // the EOS-free primitives are stored before the EOS call, so they are
     // not held in registers through it
     PrimPre a;
     {
         double xs[NSPEC];
         ctoprim_pre(u, a, xs);
-        const SpeciesPairs XQ = q_species(x.QX, I);
+        const SpeciesPairs XQ = cell_species(x.QX, I);
         CA(x.Q, I, NFQ, FQ_RHO, c) = a.rho;
         CA(x.Q, I, NFQ, FQ_U, c) = a.u;
         CA(x.Q, I, NFQ, FQ_V, c) = a.v;
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
     double *SP = x.SP[d], *SM = x.SM[d];
     const SpeciesPairs XP = species_pairs(SP, I, NFS, FS_X), XM = species_pairs(SM, I, NFS, FS_X);
-    const SpeciesPairs XQ = q_species(x.QX, I);
+    const SpeciesPairs XQ = cell_species(x.QX, I);
     const int sun = FS_U + d, sut = d == 0 ? FS_V : (d == 1 ? FS_W : FS_U), sutt = d == 0 ? FS_W : (d == 1 ? FS_U : FS_V);
     if (do_p) {
         double r, u_, e, pp;
@@ -410,12 +416,13 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     div1 = 0.1 * fmin(0.0, div1);
     const idx_t cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
     const double dxd = I.dx[d];
-#define AVN(n, fval) ((fval) + dxd * (div1 * (CA(x.U1, I, NVAR, n, cr) - CA(x.U1, I, NVAR, n, cl))))
+#define AVN(n, fval) ((fval) + dxd * (div1 * (CA(x.U1, I, NU1, n, cr) - CA(x.U1, I, NU1, n, cl))))
     const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
     double *FN = x.FN[d];
     const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
                        X2 = species_pairs(F2, I, NFF, FF_X), X3 = species_pairs(F3, I, NFF, FF_X),
                        XN = species_pairs(FN, I, NFN, FN_X);
+    const SpeciesPairs XU = cell_species(x.U1X, I);
     const double frho = AVN(URHO, o.f_rho);
     EA(FN, I, NFN, URHO, e) = frho;
     EA(FN, I, NFN, mn, e) = AVN(mn, o.f_mn);
@@ -444,11 +451,13 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
             x1[0] = trans2_species(rho[1], s.x, a.x, b.x, c3.x, d3.x, h2, h3, rrnew[1]);
             if (two) x1[1] = trans2_species(rho[1], s.y, a.y, b.y, c3.y, d3.y, h2, h3, rrnew[1]);
         }
+        const dbl2 ur = SP2(XU, pr, cr), ul = SP2(XU, pr, cl);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int n = 2 * pr + h;
             if (n >= NSPEC) break;
-            fs[n] = AVN(UFS + n, o.f_rho * (o.fpl * x0[h] + o.fmr * x1[h]));
+            const double f = o.f_rho * (o.fpl * x0[h] + o.fmr * x1[h]);
+            fs[n] = f + dxd * (div1 * ((h == 0 ? ur.x : ur.y) - (h == 0 ? ul.x : ul.y)));
             sum = sum + fs[n];
         }
     }
@@ -698,7 +707,8 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
     memset(&x, 0, sizeof(x));
     x.inst = F->inst;
     A(x.U, NVAR * offD);
-    A(x.U1, NVAR * offD);
+    A(x.U1, NU1 * offD);
+    A(x.U1X, 2L * NSPEC_PAIRS * offQ);
     A(x.Q, NFQ * offD);
     A(x.QX, 2L * NSPEC_PAIRS * offQ);
     A(x.FL, offE);
@@ -774,7 +784,7 @@ void fused_destroy(FusedLevel *F)
     if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
     if (F->graph) (void)hipGraphDestroy(F->graph);
     Ctx &x = F->ctx;
-    double *ps[] = {x.U, x.U1, x.Q, x.QX, x.FL, x.DV, x.dt_cur, x.dt_est, x.time};
+    double *ps[] = {x.U, x.U1, x.U1X, x.Q, x.QX, x.FL, x.DV, x.dt_cur, x.dt_est, x.time};
     for (double *p : ps)
         if (p) (void)hipFree(p);
     for (int d = 0; d < 3; ++d) {
@@ -813,7 +823,8 @@ int fused_export(FusedLevel *F, double *arena, hipStream_t s)
 template <class Fn>
 static void for_each_intermediate(FusedArrays &S, long totD, long totE, long totQ, Fn f)
 {
-    f(S.U1, NVAR * totD);
+    f(S.U1, NU1 * totD);
+    f(S.U1X, 2L * NSPEC_PAIRS * totQ);
     f(S.Q, NFQ * totD);
     f(S.QX, 2L * NSPEC_PAIRS * totQ);
     f(S.FL, totE);
@@ -861,6 +872,7 @@ FusedArrays fused_get_intermediates(const FusedLevel *F)
     const Ctx &x = F->ctx;
     FusedArrays S;
     S.U1 = x.U1;
+    S.U1X = x.U1X;
     S.Q = x.Q;
     S.QX = x.QX;
     S.FL = x.FL;
@@ -879,6 +891,7 @@ void fused_set_intermediates(FusedLevel *F, const FusedArrays &S)
 {
     Ctx &x = F->ctx;
     x.U1 = S.U1;
+    x.U1X = S.U1X;
     x.Q = S.Q;
     x.QX = S.QX;
     x.FL = S.FL;

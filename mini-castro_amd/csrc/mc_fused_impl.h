@@ -11,6 +11,9 @@
 
 namespace mc {
 
+// the cleaned state U1 holds the NU1 components before the species in the
+// cell layout, its species live in their own array of pairs (Ctx::U1X)
+constexpr int NU1 = UFS;
 // primitive components of the fused path; the species of q live in their
 // own array of pairs (Ctx::QX)
 enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_C, FQ_G, NFQ };
@@ -47,7 +50,7 @@ struct Plan {
 
 struct Ctx {
     const FInst *inst;
-    double *U, *U1, *Q, *QX, *FL, *DV;
+    double *U, *U1, *U1X, *Q, *QX, *FL, *DV;
     double *SM[3], *SP[3], *FD[3], *FT[6], *FN[3];
     double *dt_cur, *dt_est, *time;
     int *steps;
@@ -161,8 +164,9 @@ __device__ __forceinline__ SpeciesPairs species_pairs(double *a, const FInst &I,
 {
     return SpeciesPairs{a + ((long)C * I.offE + (long)cX * I.nE), 2 * I.nE};
 }
-// the species of the primitives q, over the instance's cells
-__device__ __forceinline__ SpeciesPairs q_species(double *QX, const FInst &I)
+// the species of a cell array kept apart in pairs (q, U1), over the
+// instance's cells
+__device__ __forceinline__ SpeciesPairs cell_species(double *QX, const FInst &I)
 {
     return SpeciesPairs{QX + 2L * NSPEC_PAIRS * I.offQ, 2 * I.nQ};
 }

@@ -111,21 +111,21 @@ __global__ __launch_bounds__(256) void A_update(Plan p, Ctx x, ACtx a)
     for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
         const dbl2 f0 = SP2(P0, pr, e), f0x = SP2(P0, pr, ex), f1 = SP2(P1, pr, e), f1y = SP2(P1, pr, ey),
                    f2 = SP2(P2, pr, e), f2z = SP2(P2, pr, ez);
-        const int n = FN_X + 2 * pr;
+        const int n = UFS + 2 * pr;
         u1[n] = ua[n] + dt * flux_div(f0.x, f0x.x, f1.x, f1y.x, f2.x, f2z.x, a0, a1, a2, volinv);
         if (n + 1 < NVAR) u1[n + 1] = ua[n + 1] + dt * flux_div(f0.y, f0x.y, f1.y, f1y.y, f2.y, f2z.y, a0, a1, a2, volinv);
     }
 #pragma unroll
-    for (int n = 0; n < FN_X; ++n) {
-        double s = flux_div(EA(F0, I, NFN, n, e), EA(F0, I, NFN, n, ex), EA(F1, I, NFN, n, e), EA(F1, I, NFN, n, ey),
-                            EA(F2, I, NFN, n, e), EA(F2, I, NFN, n, ez), a0, a1, a2, volinv);
+    for (int n = 0; n < UFS; ++n) {
+        double s = flux_div(EF(F0, I, NFN, n, e), EF(F0, I, NFN, n, ex), EF(F1, I, NFN, n, e), EF(F1, I, NFN, n, ey),
+                            EF(F2, I, NFN, n, e), EF(F2, I, NFN, n, ez), a0, a1, a2, volinv);
         if (n == UEINT) {
-            const double pdivu = HALF * (EA(F0, I, NFN, FN_GP, ex) + EA(F0, I, NFN, FN_GP, e)) *
-                                     (EA(F0, I, NFN, FN_GU, ex) - EA(F0, I, NFN, FN_GU, e)) / dx0 +
-                                 HALF * (EA(F1, I, NFN, FN_GP, ey) + EA(F1, I, NFN, FN_GP, e)) *
-                                     (EA(F1, I, NFN, FN_GU, ey) - EA(F1, I, NFN, FN_GU, e)) / dx1 +
-                                 HALF * (EA(F2, I, NFN, FN_GP, ez) + EA(F2, I, NFN, FN_GP, e)) *
-                                     (EA(F2, I, NFN, FN_GU, ez) - EA(F2, I, NFN, FN_GU, e)) / dx2;
+            const double pdivu = HALF * (EF(F0, I, NFN, FN_GP, ex) + EF(F0, I, NFN, FN_GP, e)) *
+                                     (EF(F0, I, NFN, FN_GU, ex) - EF(F0, I, NFN, FN_GU, e)) / dx0 +
+                                 HALF * (EF(F1, I, NFN, FN_GP, ey) + EF(F1, I, NFN, FN_GP, e)) *
+                                     (EF(F1, I, NFN, FN_GU, ey) - EF(F1, I, NFN, FN_GU, e)) / dx1 +
+                                 HALF * (EF(F2, I, NFN, FN_GP, ez) + EF(F2, I, NFN, FN_GP, e)) *
+                                     (EF(F2, I, NFN, FN_GU, ez) - EF(F2, I, NFN, FN_GU, e)) / dx2;
             s = s - pdivu;
         }
         u1[n] = ua[n] + dt * s;
@@ -160,8 +160,8 @@ __global__ __launch_bounds__(256) void A_update(Plan p, Ctx x, ACtx a)
         }
         // pdivu term of rho e: s -= 0.5 (GP+ + GP-) (GU+ - GU-) / dx_d
         double *F = x.FN[d];
-        const double gpp = EA(F, I, NFN, FN_GP, ep[d]), gpm = EA(F, I, NFN, FN_GP, e);
-        const double gup = EA(F, I, NFN, FN_GU, ep[d]), gum = EA(F, I, NFN, FN_GU, e);
+        const double gpp = EF(F, I, NFN, FN_GP, ep[d]), gpm = EF(F, I, NFN, FN_GP, e);
+        const double gup = EF(F, I, NFN, FN_GU, ep[d]), gum = EF(F, I, NFN, FN_GU, e);
         const double b = -sb[HB_EI] / I.dx[d];
         aadd(EAp(FB, I, NNB, NB_GP, ep[d]), b * HALF * (gup - gum));
         aadd(EAp(FB, I, NNB, NB_GP, e), b * HALF * (gup - gum));

@@ -218,26 +218,26 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     double *SP = x.SP[d], *SM = x.SM[d];
     const SpeciesPairs XP = species_pairs(SP, I, NFS, FS_X), XM = species_pairs(SM, I, NFS, FS_X);
     const SpeciesPairs XQ = cell_species(x.QX, I);
-    const int sun = FS_U + d, sut = d == 0 ? FS_V : (d == 1 ? FS_W : FS_U), sutt = d == 0 ? FS_W : (d == 1 ? FS_U : FS_V);
+    // velocities back in absolute order: the normal one is FS_U + d, the
+    // first transverse FS_V, FS_W, FS_U and the second FS_W, FS_U, FS_V for
+    // d = 0, 1, 2; stored as the pairs (rho, u), (v, w), (p, rhoe)
+    auto store_state = [&](double *S, idx_t ee, double r, double un_, double ut_, double utt_, double e_, double p_) {
+        const double vx = d == 0 ? un_ : (d == 1 ? utt_ : ut_);
+        const double vy = d == 0 ? ut_ : (d == 1 ? un_ : utt_);
+        const double vz = d == 0 ? utt_ : (d == 1 ? ut_ : un_);
+        EF2(S, I, NFS, FS_RHO, ee) = dbl2{r, vx};
+        EF2(S, I, NFS, FS_V, ee) = dbl2{vy, vz};
+        EF2(S, I, NFS, FS_P, ee) = dbl2{p_, e_};
+    };
     if (do_p) {
         double r, u_, e, pp;
         ppm_project<true>(Im, gam_m, un, cc, r, u_, e, pp);
-        EA(SP, I, NFS, FS_RHO, ep) = r;
-        EA(SP, I, NFS, sun, ep) = u_;
-        EA(SP, I, NFS, FS_RE, ep) = e;
-        EA(SP, I, NFS, FS_P, ep) = pp;
-        EA(SP, I, NFS, sut, ep) = Im_ut;
-        EA(SP, I, NFS, sutt, ep) = Im_utt;
+        store_state(SP, ep, r, u_, Im_ut, Im_utt, e, pp);
     }
     if (do_m) {
         double r, u_, e, pp;
         ppm_project<false>(Ip, gam_p, un, cc, r, u_, e, pp);
-        EA(SM, I, NFS, FS_RHO, em) = r;
-        EA(SM, I, NFS, sun, em) = u_;
-        EA(SM, I, NFS, FS_RE, em) = e;
-        EA(SM, I, NFS, FS_P, em) = pp;
-        EA(SM, I, NFS, sut, em) = Ip_ut;
-        EA(SM, I, NFS, sutt, em) = Ip_utt;
+        store_state(SM, em, r, u_, Ip_ut, Ip_utt, e, pp);
     }
     // species last: their stores come after every read of the instance table
     for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
@@ -424,13 +424,17 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
                        XN = species_pairs(FN, I, NFN, FN_X);
     const SpeciesPairs XU = cell_species(x.U1X, I);
     const double frho = AVN(URHO, o.f_rho);
-    EA(FN, I, NFN, URHO, e) = frho;
-    EA(FN, I, NFN, mn, e) = AVN(mn, o.f_mn);
-    EA(FN, I, NFN, mt, e) = AVN(mt, o.f_mt);
-    EA(FN, I, NFN, mtt, e) = AVN(mtt, o.f_mtt);
-    EA(FN, I, NFN, UEDEN, e) = AVN(UEDEN, o.f_E);
-    EA(FN, I, NFN, UEINT, e) = AVN(UEINT, o.f_e);
-    EA(FN, I, NFN, UTEMP, e) = 0.0;
+    {
+        // momenta back in absolute order (mn = UMX + d, mt, mtt as above)
+        const double vmn = AVN(mn, o.f_mn), vmt = AVN(mt, o.f_mt), vmtt = AVN(mtt, o.f_mtt);
+        const double mx = d == 0 ? vmn : vmt;
+        const double my = d == 0 ? vmt : (d == 1 ? vmn : vmtt);
+        const double mz = d == 2 ? vmn : vmtt;
+        EF2(FN, I, NFN, URHO, e) = dbl2{frho, mx};
+        EF2(FN, I, NFN, UMY, e) = dbl2{my, mz};
+        EF2(FN, I, NFN, UEDEN, e) = dbl2{AVN(UEDEN, o.f_E), AVN(UEINT, o.f_e)};
+        EF2(FN, I, NFN, UTEMP, e) = dbl2{0.0, o.g_p};  // UTEMP, FN_GP
+    }
     double fs[NSPEC], sum = 0.0;
     const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
 #pragma unroll
@@ -466,8 +470,7 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 #pragma unroll
     for (int pr = 0; pr < NSPEC_PAIRS; ++pr)
         SP2(XN, pr, e) = dbl2{fs[2 * pr] * fac, 2 * pr + 1 < NSPEC ? fs[2 * pr + 1] * fac : 0.0};
-    EA(FN, I, NFN, FN_GP, e) = o.g_p;
-    EA(FN, I, NFN, FN_GU, e) = o.g_un;
+    EF(FN, I, NFN, FN_GU, e) = o.g_un;
 }
 
 // fill_hydro_source (Source/Castro_hydro_nd.F90:344-441), Saxpy and the two
@@ -501,25 +504,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
         for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
             const dbl2 f0 = SP2(P0, pr, e), f0x = SP2(P0, pr, ex), f1 = SP2(P1, pr, e), f1y = SP2(P1, pr, ey),
                        f2 = SP2(P2, pr, e), f2z = SP2(P2, pr, ez);
-            const int n = FN_X + 2 * pr;
+            const int n = UFS + 2 * pr;
             u[n] = u[n] + dt * flux_div(f0.x, f0x.x, f1.x, f1y.x, f2.x, f2z.x, a0, a1, a2, volinv);
             if (n + 1 < NVAR) u[n + 1] = u[n + 1] + dt * flux_div(f0.y, f0x.y, f1.y, f1y.y, f2.y, f2z.y, a0, a1, a2, volinv);
         }
+        // URHO..UTEMP and FN_GP as the four pairs (0,1) (2,3) (4,5) (6,7) of
+        // the six faces; the last pair (UTEMP, Godunov p) first, for p div u
+        double pdivu;
+        {
+            const dbl2 g0 = EF2(F0, I, NFN, UTEMP, e), g0x = EF2(F0, I, NFN, UTEMP, ex), g1 = EF2(F1, I, NFN, UTEMP, e),
+                       g1y = EF2(F1, I, NFN, UTEMP, ey), g2 = EF2(F2, I, NFN, UTEMP, e),
+                       g2z = EF2(F2, I, NFN, UTEMP, ez);
+            u[UTEMP] = u[UTEMP] + dt * flux_div(g0.x, g0x.x, g1.x, g1y.x, g2.x, g2z.x, a0, a1, a2, volinv);
+            pdivu = HALF * (g0x.y + g0.y) * (EF(F0, I, NFN, FN_GU, ex) - EF(F0, I, NFN, FN_GU, e)) / dx0 +
+                    HALF * (g1y.y + g1.y) * (EF(F1, I, NFN, FN_GU, ey) - EF(F1, I, NFN, FN_GU, e)) / dx1 +
+                    HALF * (g2z.y + g2.y) * (EF(F2, I, NFN, FN_GU, ez) - EF(F2, I, NFN, FN_GU, e)) / dx2;
+        }
 #pragma unroll
-        for (int n = 0; n < FN_X; ++n) {
-            double s = flux_div(EA(F0, I, NFN, n, e), EA(F0, I, NFN, n, ex), EA(F1, I, NFN, n, e),
-                                EA(F1, I, NFN, n, ey), EA(F2, I, NFN, n, e), EA(F2, I, NFN, n, ez), a0, a1, a2, volinv);
-            if (n == UEINT) {
-                const double pdivu =
-                    HALF * (EA(F0, I, NFN, FN_GP, ex) + EA(F0, I, NFN, FN_GP, e)) *
-                        (EA(F0, I, NFN, FN_GU, ex) - EA(F0, I, NFN, FN_GU, e)) / dx0 +
-                    HALF * (EA(F1, I, NFN, FN_GP, ey) + EA(F1, I, NFN, FN_GP, e)) *
-                        (EA(F1, I, NFN, FN_GU, ey) - EA(F1, I, NFN, FN_GU, e)) / dx1 +
-                    HALF * (EA(F2, I, NFN, FN_GP, ez) + EA(F2, I, NFN, FN_GP, e)) *
-                        (EA(F2, I, NFN, FN_GU, ez) - EA(F2, I, NFN, FN_GU, e)) / dx2;
-                s = s - pdivu;
-            }
-            u[n] = u[n] + dt * s;
+        for (int n = 0; n < UTEMP; n += 2) {
+            const dbl2 f0 = EF2(F0, I, NFN, n, e), f0x = EF2(F0, I, NFN, n, ex), f1 = EF2(F1, I, NFN, n, e),
+                       f1y = EF2(F1, I, NFN, n, ey), f2 = EF2(F2, I, NFN, n, e), f2z = EF2(F2, I, NFN, n, ez);
+            const double s0 = flux_div(f0.x, f0x.x, f1.x, f1y.x, f2.x, f2z.x, a0, a1, a2, volinv);
+            double s1 = flux_div(f0.y, f0x.y, f1.y, f1y.y, f2.y, f2z.y, a0, a1, a2, volinv);
+            if (n + 1 == UEINT) s1 = s1 - pdivu;
+            u[n] = u[n] + dt * s0;
+            u[n + 1] = u[n + 1] + dt * s1;
         }
         clean_cell(u, x.tab); /* Castro_advance.cpp:78 */
         clean_cell(u, x.tab); /* post_timestep, Castro.cpp:369 */
@@ -679,8 +688,8 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
         I.nD = (long)I.n[0] * I.n[1] * I.n[2];
         I.nE = (long)I.ext[0] * I.ext[1] * I.ext[2];
         I.nE += I.nE & 1;  // even: the species pairs (SP2) stay 16-byte aligned
-        if (I.nE >= (1L << 29)) {  // per-instance byte offsets are 32-bit
-            set_error("fused engine: an instance may hold at most 2^29 extended cells");
+        if (I.nE >= (1L << 28)) {  // 16-byte lane offsets of the face pairs are 32-bit
+            set_error("fused engine: an instance may hold at most 2^28 extended cells");
             delete F;
             return nullptr;
         }

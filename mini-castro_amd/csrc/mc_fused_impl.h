@@ -17,16 +17,23 @@ constexpr int NU1 = UFS;
 // primitive components of the fused path; the species of q live in their
 // own array of pairs (Ctx::QX)
 enum { FQ_RHO = 0, FQ_U, FQ_V, FQ_W, FQ_P, FQ_RE, FQ_C, FQ_G, NFQ };
-// The passive species of the state and flux arrays are stored in pairs
-// (SP2 below): 2 * NSPEC_PAIRS component slots for NSPEC species.
+// The face arrays below store their components in interleaved pairs (EF,
+// EF2): 2 * NSPEC_PAIRS component slots for NSPEC species, every pair
+// starting at an even component and every component count even.
 constexpr int NSPEC_PAIRS = (NSPEC + 1) / 2;
 // traced / corrected interface states (velocities in absolute order)
 enum { FS_RHO = 0, FS_U, FS_V, FS_W, FS_P, FS_RE, FS_X, NFS = FS_X + 2 * NSPEC_PAIRS };
-// transverse-stage fluxes: conserved URHO..UEDEN, species, Godunov p and normal u
-enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_X, FF_GP = FF_X + 2 * NSPEC_PAIRS, FF_GU, NFF };
-// final fluxes: NVAR conserved components (species in pairs), Godunov p and normal u
+// transverse-stage fluxes: conserved URHO..UEDEN, Godunov p, species,
+// Godunov normal u (and a padding slot)
+enum { FF_RHO = 0, FF_MX, FF_MY, FF_MZ, FF_E, FF_GP, FF_X, FF_GU = FF_X + 2 * NSPEC_PAIRS, NFF = FF_GU + 2 };
+// final fluxes: conserved URHO..UTEMP, Godunov p, species, Godunov normal u
+// (and a padding slot)
 static_assert(UFS + NSPEC == NVAR, "the species are the last conserved components");
-enum { FN_X = UFS, FN_GP = FN_X + 2 * NSPEC_PAIRS, FN_GU, NFN };
+enum { FN_GP = UFS, FN_X, FN_GU = FN_X + 2 * NSPEC_PAIRS, NFN = FN_GU + 2 };
+static_assert(NFS % 2 == 0 && NFF % 2 == 0 && NFN % 2 == 0, "face arrays hold whole pairs");
+static_assert(FS_X % 2 == 0 && FF_X % 2 == 0 && FN_X % 2 == 0, "species pairs start at even components");
+static_assert(FS_RHO == 0 && FS_U == 1 && FS_V == 2 && FS_W == 3 && FS_P == 4 && FS_RE == 5, "state pairs");
+static_assert(FF_E == UEDEN && FF_GP == FF_E + 1 && FN_GP == UTEMP + 1, "flux pairs");
 
 struct FInst {
     int n[3];
@@ -113,8 +120,8 @@ __device__ __forceinline__ int bmap(int i, int n, int periodic)
 }
 
 // Cell and face indices within one instance are 32-bit and so are the byte
-// offsets of an instance's component arrays (fewer than 2^29 extended cells
-// per instance, checked at creation).  The array base of a component is
+// offsets of an instance's component arrays (fewer than 2^28 extended cells
+// per instance, checked at creation, so even 16-byte pair offsets fit).  The array base of a component is
 // uniform across a workgroup (one instance per workgroup), so every access is
 // a scalar base plus a 32-bit per-lane byte offset.
 typedef unsigned idx_t;
@@ -144,18 +151,36 @@ __device__ __forceinline__ gdouble &at(double *base, idx_t idx)
     return *(gdouble *)(g + idx * 8u);
 }
 
-// The species of a state or flux array (C components per element, species
-// block at component cX) in pairs: species 2p and 2p+1 share components
-// cX+2p and cX+2p+1 as one interleaved region, element idx of species n at
-// double offset (cX + 2*(n/2))*nE + 2*idx + n%2 of the instance's part.  One
-// 16-byte access per lane moves both species of a pair, half the load and
-// store instructions of separate components.  nE and the instance offsets
-// are even, so the regions are 16-byte aligned.  The base and pair stride
-// are taken once, before a kernel's species loop, so that the loop's stores
-// (which may alias the instance table as far as the compiler knows) do not
-// force the instance fields to be loaded again in every pass.
+// Face arrays (traced states SM/SP, fluxes F^d and F^{t|d}, final fluxes)
+// hold their C components in interleaved pairs: components 2r and 2r+1
+// share one region of 2*nE doubles, element idx of component c at double
+// offset (c & ~1)*nE + 2*idx + (c & 1) of the instance's part.  One 16-byte
+// access per lane moves both components of a pair (EF2), half the load and
+// store instructions of separate components; EF reaches one component.  C,
+// nE and the instance offsets are even, so the pairs are 16-byte aligned,
+// and nE < 2^28 keeps the 16-byte lane offset in 32 bits.
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) dbl2 gdbl2;
+__device__ __forceinline__ gdouble &EF(double *a, const FInst &I, int C, int c, idx_t idx)
+{
+    typedef __attribute__((address_space(1))) char gchar;
+    gchar *g = (gchar *)uniform_ptr(a + ((long)C * I.offE + (long)(c & ~1) * I.nE + (c & 1)));
+    return *(gdouble *)(g + idx * 16u);
+}
+// components c (even) and c+1
+__device__ __forceinline__ gdbl2 &EF2(double *a, const FInst &I, int C, int c, idx_t idx)
+{
+    typedef __attribute__((address_space(1))) char gchar;
+    gchar *g = (gchar *)uniform_ptr(a + ((long)C * I.offE + (long)c * I.nE));
+    return *(gdbl2 *)(g + idx * 16u);
+}
+
+// The species of a face array (species block at the even component cX) as
+// NSPEC_PAIRS pairs, and the species kept apart in pairs of a cell array
+// (q, U1; count nQ).  The base and pair stride are taken once, before a
+// kernel's species loop, so that the loop's stores (which may alias the
+// instance table as far as the compiler knows) do not force the instance
+// fields to be loaded again in every pass.
 struct SpeciesPairs {
     double *base;
     long stride;  // doubles from one pair region to the next
@@ -170,12 +195,12 @@ __device__ __forceinline__ SpeciesPairs cell_species(double *QX, const FInst &I)
 {
     return SpeciesPairs{QX + 2L * NSPEC_PAIRS * I.offQ, 2 * I.nQ};
 }
-// pair p at element idx; the lane offset is 64-bit (it can pass 2^32 bytes)
+// pair p at element idx (idx < 2^28)
 __device__ __forceinline__ gdbl2 &SP2(const SpeciesPairs &S, int p, idx_t idx)
 {
     typedef __attribute__((address_space(1))) char gchar;
     gchar *g = (gchar *)uniform_ptr(S.base + p * S.stride);
-    return *(gdbl2 *)(g + (unsigned long long)idx * 16u);
+    return *(gdbl2 *)(g + idx * 16u);
 }
 
 // Flux divergence of one conserved component (Source/Castro_hydro_nd.F90:
@@ -215,23 +240,24 @@ __device__ __forceinline__ gdouble &EA(double *a, const FInst &I, int C, int c, 
 // Hydro part (rho, velocities permuted to direction d, p, rhoe) of a state
 __device__ __forceinline__ void load_fs_h(double *S, const FInst &I, idx_t e, int d, FaceH &f)
 {
-    const int un = FS_U + d, ut = d == 0 ? FS_V : FS_U, utt = d == 2 ? FS_V : FS_W;
-    f.rho = EA(S, I, NFS, FS_RHO, e);
-    f.un = EA(S, I, NFS, un, e);
-    f.ut = EA(S, I, NFS, ut, e);
-    f.utt = EA(S, I, NFS, utt, e);
-    f.p = EA(S, I, NFS, FS_P, e);
-    f.rhoe = EA(S, I, NFS, FS_RE, e);
+    const dbl2 ru = EF2(S, I, NFS, FS_RHO, e), vw = EF2(S, I, NFS, FS_V, e), pe = EF2(S, I, NFS, FS_P, e);
+    f.rho = ru.x;
+    f.un = d == 0 ? ru.y : (d == 1 ? vw.x : vw.y);  // FS_U + d
+    f.ut = d == 0 ? vw.x : ru.y;                     // FS_V, else FS_U
+    f.utt = d == 2 ? vw.x : vw.y;                    // FS_V at d = 2, else FS_W
+    f.p = pe.x;
+    f.rhoe = pe.y;
 }
 
 __device__ __forceinline__ void load_qin_h(double *S, const FInst &I, idx_t e, QInH &q)
 {
-    q.rho = EA(S, I, NFS, FS_RHO, e);
-    q.u = EA(S, I, NFS, FS_U, e);
-    q.v = EA(S, I, NFS, FS_V, e);
-    q.w = EA(S, I, NFS, FS_W, e);
-    q.p = EA(S, I, NFS, FS_P, e);
-    q.rhoe = EA(S, I, NFS, FS_RE, e);
+    const dbl2 ru = EF2(S, I, NFS, FS_RHO, e), vw = EF2(S, I, NFS, FS_V, e), pe = EF2(S, I, NFS, FS_P, e);
+    q.rho = ru.x;
+    q.u = ru.y;
+    q.v = vw.x;
+    q.w = vw.y;
+    q.p = pe.x;
+    q.rhoe = pe.y;
     q.game = 0.0;
 }
 
@@ -250,14 +276,15 @@ __device__ __forceinline__ FaceH to_face(const QStateH &o, int d)
 // Non-species part of a transverse-stage flux; momenta in absolute order.
 __device__ __forceinline__ void store_tflux_h(double *F, const FInst &I, idx_t e, int d, const RiemannH &o)
 {
-    const int mn = FF_MX + d, mt = d == 0 ? FF_MY : FF_MX, mtt = d == 2 ? FF_MY : FF_MZ;
-    EA(F, I, NFF, FF_RHO, e) = o.f_rho;
-    EA(F, I, NFF, mn, e) = o.f_mn;
-    EA(F, I, NFF, mt, e) = o.f_mt;
-    EA(F, I, NFF, mtt, e) = o.f_mtt;
-    EA(F, I, NFF, FF_E, e) = o.f_E;
-    EA(F, I, NFF, FF_GP, e) = o.g_p;
-    EA(F, I, NFF, FF_GU, e) = o.g_un;
+    // momentum FF_MX + d is the normal one; then FF_MY (d = 0) or FF_MX, then
+    // FF_MY (d = 2) or FF_MZ
+    const double mx = d == 0 ? o.f_mn : o.f_mt;
+    const double my = d == 0 ? o.f_mt : (d == 1 ? o.f_mn : o.f_mtt);
+    const double mz = d == 2 ? o.f_mn : o.f_mtt;
+    EF2(F, I, NFF, FF_RHO, e) = dbl2{o.f_rho, mx};
+    EF2(F, I, NFF, FF_MY, e) = dbl2{my, mz};
+    EF2(F, I, NFF, FF_E, e) = dbl2{o.f_E, o.g_p};
+    EF(F, I, NFF, FF_GU, e) = o.g_un;
 }
 
 // Riemann problem on face (i,j,k) of direction d (cells i-e_d and i)
@@ -273,10 +300,14 @@ __device__ __forceinline__ void face_riemann_h(const Ctx &x, const FInst &I, int
 // URHO..UEDEN of a transverse-stage flux and its Godunov p, u
 __device__ __forceinline__ void load_tflux_h(double *F, const FInst &I, idx_t e, double *v, double &gp, double &gu)
 {
-#pragma unroll
-    for (int n = 0; n <= UEDEN; ++n) v[n] = EA(F, I, NFF, FF_RHO + n, e);
-    gp = EA(F, I, NFF, FF_GP, e);
-    gu = EA(F, I, NFF, FF_GU, e);
+    const dbl2 a = EF2(F, I, NFF, FF_RHO, e), b = EF2(F, I, NFF, FF_MY, e), c = EF2(F, I, NFF, FF_E, e);
+    v[URHO] = a.x;
+    v[UMX] = a.y;
+    v[UMY] = b.x;
+    v[UMZ] = b.y;
+    v[UEDEN] = c.x;
+    gp = c.y;
+    gu = EF(F, I, NFF, FF_GU, e);
 }
 
 static __constant__ int c_pair_t[6] = {1, 2, 0, 2, 0, 1};   // state (face) direction

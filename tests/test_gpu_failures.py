@@ -89,9 +89,10 @@ def test_non_cubic_cells_rejected():
 
 
 def test_oversized_instance_rejected_before_allocation():
-    """The fused engine addresses an instance with 32-bit byte offsets and
-    so caps it below 2^29 extended cells: an 812^3 instance (814^3 > 2^29)
-    is refused with an error, before any device memory is taken."""
-    with pytest.raises(RuntimeError, match="2\\^29"):
+    """The fused engine addresses an instance's face pairs with 32-bit
+    16-byte lane offsets and so caps it below 2^28 extended cells: an 812^3
+    instance (814^3 > 2^28) is refused with an error, before any device
+    memory is taken."""
+    with pytest.raises(RuntimeError, match="2\\^28"):
         mc.Ensemble([dict(n_cell=(812, 812, 812), prob_lo=(0, 0, 0), prob_hi=(1, 1, 1), bc=1,
                           p_ambient=1e13, dens_ambient=1.0, exp_energy=1e12, r_init=0.1)])

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
 
 // Source/riemann.F90:10-331 on the PPM states: F^d on d-faces
 // (d-face index 0..n_d, transverse cells -1..n).  Species are streamed.
-__global__ __launch_bounds__(256) void K_flux1(Plan p, Ctx x)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K_flux1(Plan p, Ctx x)
 {
     int m, i, j, k, sub_;
     if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;

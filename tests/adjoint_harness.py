@@ -73,7 +73,10 @@ class SedovAdjointProblem:
         dx = (hi - lo) / np.asarray(n)
         ctr = 0.5 * (lo + hi)
         xs = [lo[d] + dx[d] * (np.arange(n[d]) + 0.5) - ctr[d] for d in range(3)]
-        r = np.sqrt(xs[2][:, None, None] ** 2 + xs[1][None, :, None] ** 2 + xs[0][None, None, :] ** 2)
+        # x^2 + y^2 + z^2 in the oracle harness's order: at odd sizes some
+        # cells lie exactly on the shell radius, and the rounding of the sum
+        # decides on which side
+        r = np.sqrt(xs[0][None, None, :] ** 2 + xs[1][None, :, None] ** 2 + xs[2][:, None, None] ** 2)
         r0, r1 = cfg.get("shell", (0.2, 0.3))
         self.mask = torch.from_numpy(((r >= r0) & (r <= r1)).astype(np.float64)).to("cuda")
         self.adj = mc.Adjoint(n, cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["steps"])

@@ -160,6 +160,13 @@ def test_adjoint_periodic_matches_cpu_tangent_linear():
     assert max(tangent_check(16, 4, 4, seed=23, bc=0, r_init=0.3, shell=(0.25, 0.5))) < 1e-10
 
 
+def test_adjoint_odd_size_matches_cpu_tangent_linear():
+    """A 15^3 problem: odd cell and extended counts, so the engine's pair
+    arrays carry a padding element and the adjoint's own buffers must use
+    the engine's padded stride."""
+    assert max(tangent_check(15, 3, 4, seed=29)) < 1e-10
+
+
 @pytest.mark.parametrize("n,steps", [(16, 3), (64, 10)])
 def test_kept_and_recomputed_intermediates_agree(n, steps):
     """The reverse pass reads each step's intermediates as the forward pass

@@ -103,12 +103,37 @@ def test_autotune_keeps_state_and_results():
     np.testing.assert_array_equal(A.state(), B.state())
 
 
-def test_ensemble_matches_sequential_runs():
+def test_fused_odd_domain_matches_oracle():
+    """A 15^3 domain in ragged 8/7-cell boxes: its cell count (3375) and
+    extended count (17^3) are odd, so the engine's even-padded pair arrays
+    carry a padding element; the state must still equal the oracle's bit for
+    bit, outflow and periodic."""
+    for bc in (1, 0):
+        cfg = small_cfg(n=15, bc=bc, r_init=0.14)
+        O = oracle_level(cfg, nthreads=8)
+        G = gpu_level(cfg, boxes=configs.grid_boxes(15, 8))
+        for s in range(3):
+            dto, dtg = O.coarse_step(), G.coarse_step()
+            assert dtg == dto, (bc, s, dtg, dto)
+        np.testing.assert_array_equal(G.state(), O.state(), err_msg="bc %d" % bc)
+
+
+ENSEMBLE_SETS = {
+    "even": ((16, 2e12, 0.12, 0.7, 1), (24, 5e12, 0.09, 1.6, 0), (20, 1e12, 0.15, 1.1, 1)),
+    # odd sides: every instance's cell and extended counts are odd, so each
+    # instance offset in the pair arrays includes the padding of the ones
+    # before it
+    "odd": ((15, 2e12, 0.12, 0.7, 1), (13, 5e12, 0.09, 1.6, 0), (17, 1e12, 0.15, 1.1, 1), (11, 3e12, 0.2, 0.9, 0)),
+}
+
+
+@pytest.mark.parametrize("sizes", sorted(ENSEMBLE_SETS))
+def test_ensemble_matches_sequential_runs(sizes):
     """Instances of different sizes and constants advanced together, each with
     its own dt, equal the same instances advanced one at a time by the per-box
     path: every state bit and every instance's dt."""
     insts = []
-    for side, E, r, rho, bc in ((16, 2e12, 0.12, 0.7, 1), (24, 5e12, 0.09, 1.6, 0), (20, 1e12, 0.15, 1.1, 1)):
+    for side, E, r, rho, bc in ENSEMBLE_SETS[sizes]:
         d = small_cfg(n=side, bc=bc, r_init=r, exp_energy=E, dens_ambient=rho)
         insts.append(d)
     ens = mc.Ensemble(insts)

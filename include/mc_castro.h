@@ -287,7 +287,11 @@ void mc_advance_batch_release(void);
    Every instance is its own domain [0, n_cell) with its own problem constants
    and time-step control; the fused engine advances all of them with one
    launch per stage.  Instance state: 20 components, Fortran order over the
-   whole instance box (i fastest, component slowest). */
+   whole instance box (i fastest, component slowest).  The fused engine (this,
+   MC_MODE_FUSED levels, mc_advance_batch, the adjoint) takes domains with
+   fewer than 2^28 cells including one ghost layer, (nx+2)(ny+2)(nz+2) <
+   2^28, and refuses larger ones with an error (more than one GPU's memory
+   would hold in any case). */
 typedef struct mc_ensemble mc_ensemble_t;
 /* n_cell, prob_lo, prob_hi: 3 per instance; bc: one per instance */
 mc_ensemble_t *mc_ensemble_create(int ninst, const int *n_cell, const double *prob_lo, const double *prob_hi,

@@ -33,8 +33,11 @@
 //
 // Layout: per instance, structure-of-arrays; a cell array with C components
 // holds component c of cell (i,j,k) at C*off + c*ncell + i + n0*(j + n1*k);
-// extended arrays cover [-1, n] in each direction.  dt lives on the device
-// (one slot per instance), so a step can be captured in a hipGraph.
+// extended arrays cover [-1, n] in each direction.  The face arrays hold
+// their components in interleaved pairs, and the species of q and of the
+// cleaned state sit apart in arrays of pairs (EF, EF2, SP2 in
+// mc_fused_impl.h), so one 16-byte access moves two components.  dt lives on
+// the device (one slot per instance), so a step can be captured in a hipGraph.
 #include <float.h>
 #include <stdio.h>
 #include <string.h>
@@ -262,7 +265,8 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
 }
 
 // Source/riemann.F90:10-331 on the PPM states: F^d on d-faces
-// (d-face index 0..n_d, transverse cells -1..n).  Species are streamed.
+// (d-face index 0..n_d, transverse cells -1..n).  Species are streamed in
+// pairs.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K_flux1(Plan p, Ctx x)
 {
     int m, i, j, k, sub_;

@@ -50,7 +50,7 @@ def _stage_bytes(stage, n):
     cell, ring, face, tface, fface = _counts(n)
     NS, NF, NQ, NN = 19, 20, 21, 22   # state, flux, primitive, final-flux components
     doubles = {
-        "prim": (20 + 20 + NQ) * cell,                             # U in; U1, Q out
+        "prim": (20 + 20 + 20 + NQ) * cell,                        # U in; clean(U) in place, U1, Q out
         "flat": 4 * cell + 2 * ring,                               # p, u, v, w in; flatn, div out
         "trace": NQ * cell + ring + 3 * 2 * NS * ring,             # Q, flatn in; minus/plus states out
         "flux1": 3 * (2 * NS + NF) * face + 2 * cell,              # states in, F^d out; c, gamc in

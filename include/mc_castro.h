@@ -246,7 +246,9 @@ int mc_level_sync(mc_level_t *L);
    final, update; a brick of bx*by*bz threads (multiple of 64, <= 256). */
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
-/* Launch one fused stage on the current device state (for timing). */
+/* Launch one fused stage on the current device state (for timing).  The
+   prim stage replaces the state by its clean and the update stage advances
+   it, both in place; the other stages leave it unchanged. */
 int mc_level_run_stage(mc_level_t *L, int stage);
 /* Fused mode: time every candidate brick of every stage of the step (reps
    launches each) on the data of one coarse step and keep the fastest; the

@@ -3,7 +3,8 @@
 // All boxes of a level -- and all members of an ensemble of independent
 // Sedov instances -- advance together, one launch per stage:
 //
-//   K_prim    cells   clean_state(S_old), clean_state(Sborder), ctoprim
+//   K_prim    cells   clean_state(S_old) (stored in place), clean_state(Sborder),
+//                     ctoprim
 //   K_flat    cells+1 shock flattening coefficient, node-centred div(u)
 //   K_trace   cells+1 PPM edge states in x, y, z
 //   K_flux1   faces   CGF fluxes of the PPM states           F^d
@@ -85,8 +86,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
     double u[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
-    clean_cell(u, x.tab);
-    clean_cell(u, x.tab);
+    clean_cell(u, x.tab); /* S_old clean, Castro_advance.cpp:51 */
+    // S_old is replaced by its clean in place: K_update starts from it
+    // instead of cleaning S_old a second time (nothing between reads U)
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) CA(x.U, I, NVAR, n, c) = u[n];
+    clean_cell(u, x.tab); /* Sborder clean, Castro_advance.cpp:66 */
 #pragma unroll
     for (int n = 0; n < NU1; ++n) CA(x.U1, I, NU1, n, c) = u[n];
     {
@@ -502,8 +507,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
         const double dt = x.dt_cur[m];
         double u[NVAR];
 #pragma unroll
-        for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);
-        clean_cell(u, x.tab); /* S_old clean, Castro_advance.cpp:51 */
+        for (int n = 0; n < NVAR; ++n) u[n] = CA(x.U, I, NVAR, n, c);  // clean(S_old), stored by K_prim
 #pragma unroll
         for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
             const dbl2 f0 = SP2(P0, pr, e), f0x = SP2(P0, pr, ex), f1 = SP2(P1, pr, e), f1y = SP2(P1, pr, ey),
@@ -1111,8 +1115,9 @@ static const KernelTiling tune_cand[] = {
     {16, 16, 1}, {16, 8, 2}, {16, 4, 4}, {8, 8, 4}, {128, 2, 1}};
 
 // Tiling autotuner: times every candidate brick of every stage of the step
-// on the data one coarse step leaves behind and keeps the fastest.  No stage
-// reads an array it writes, so repeating a launch recomputes the same
+// on the data one coarse step leaves behind and keeps the fastest.  Apart
+// from the state, which K_prim (its clean) and K_update rewrite in place, no
+// stage reads an array it writes, so repeating a launch recomputes the same
 // outputs; the state and the clocks are restored afterwards, so results do
 // not change (tilings never change results).  bricks (8 x 3) and best_ms (8)
 // receive the chosen brick and its time per launch; either may be null.

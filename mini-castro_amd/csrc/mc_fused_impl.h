@@ -204,7 +204,7 @@ __device__ __forceinline__ gdbl2 &SP2(const SpeciesPairs &S, int p, idx_t idx)
 }
 
 // Flux divergence of one conserved component (Source/Castro_hydro_nd.F90:
-// 382-391): the six face fluxes times their areas, over the cell volume.
+// 418-422): the six face fluxes times their areas, over the cell volume.
 __device__ __forceinline__ double flux_div(double f0, double f0x, double f1, double f1y, double f2, double f2z,
                                            double a0, double a1, double a2, double volinv)
 {

@@ -4,17 +4,23 @@ A step is one Amr::coarseTimeStep of mini-Castro (time-step control with the
 estdt reduction, Castro::advance, post_timestep) over every box of the
 level, with the state resident in HBM.  Default workload: 256^3 Sedov tiled
 by 64 seeded uneven boxes (sides in {16,...,96}), batched fused path.  The
-line also carries the per-box (reference kernel order) rate on the same
-workload, the roofline of the dominant stage measured with HIP events on the
-stream it runs on, and the CPU oracle timed on a bounded sample.
+line also carries: the whole-step FP64 roofline against the reference
+algorithm's work per zone-update and the measured HBM traffic per step; the
+dominant stage measured with HIP events on the stream it runs on; the
+per-box (reference kernel order) rate and the rate of the drop-in batched
+entry mc_advance_batch on the same workload; the discrete adjoint; the
+ensemble; and the CPU oracle timed on one step of the same workload.
 
 Prints one JSON line.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (os.path.join(ROOT, "mini-castro_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
@@ -114,6 +120,14 @@ def measured_fp64_ops(kernel, workload):
     return None
 
 
+def log(msg):
+    """Progress line on stderr (the JSON line alone goes to stdout)."""
+    print("[bench %.0fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
+
+
+T_START = time.perf_counter()
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -125,53 +139,106 @@ def _cpu_model():
     return None
 
 
-def cpu_baseline(cfg_full, n=64, steps=2):
-    """The oracle (CPU restatement, OpenMP over (1024,16,16) tiles) on an n^3
-    single-box sample of the same problem, in zone-updates/s: the timing build
-    (-O3, AVX2/FMA) on up to 16 host threads and, for one more step from the
-    same state, on one thread; and the parity build (no FMA contraction, the
-    one the tests compare against) on the same threads."""
-    import numpy as np
+def cpu_baseline(cfg, S, dt_prev, one_core=False):
+    """The oracle (CPU restatement, OpenMP over the reference's (1024,16,16)
+    tiles) on the benchmark workload itself: one coarse step of every box of
+    the level (estTimeStep, computeNewDt from the previous step's dt,
+    Castro::advance, post_timestep), starting from the state S the GPU run
+    reached, timed with the timing build (-O3, AVX2/FMA) on up to 16 host
+    threads, in zone-updates/s.
+    one_core: the same step once more on one thread (about three minutes at
+    256^3)."""
     import orc
     from helpers import table_array
-    from minicastro import configs
     threads = max(1, min(16, os.cpu_count() or 1))
-    cfg = dict(cfg_full)
-    cfg.update(n_cell=(n, n, n), boxes=configs.grid_boxes(n, n))
-    tab = np.ascontiguousarray(table_array(), dtype=np.float64)
+    n = cfg["n_cell"]
+    zones = int(n[0]) * int(n[1]) * int(n[2])
+    lib = orc.fast_lib()
+    lib.orc_eos_init_arrays(orc.ptr(np.ascontiguousarray(table_array(), dtype=np.float64)))
 
-    def level(lib, nthreads):
-        lib.orc_eos_init_arrays(orc.ptr(tab))
-        L = orc.Level(cfg["n_cell"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["boxes"], nthreads,
-                      _library=lib)
-        L.init_data(cfg["p_ambient"], cfg["dens_ambient"], cfg["exp_energy"], cfg["r_init"], cfg["nsub"])
-        return L
-
-    def timed(L, k):
+    def timed_step(nthreads):
+        L = orc.Level(n, cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["boxes"], nthreads, _library=lib)
+        L.set_state(S)
         t0 = time.perf_counter()
-        for _ in range(k):
-            L.coarse_step()
-        return n ** 3 * k / (time.perf_counter() - t0)
-
-    rate = {}
-    for name, lib in (("fast", orc.fast_lib()), ("parity", orc.lib())):
-        L = level(lib, threads)
-        L.coarse_step()  # first step (the initial-temperature Newton solves are longer); untimed
-        rate[name] = timed(L, steps)
-        if name == "fast":
-            one = orc.Level(cfg["n_cell"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["boxes"], 1,
-                            _library=lib)
-            one.set_state(L.state())
-            rate["fast_1core"] = timed(one, 1)
-            del one
+        dt = min(L.est_time_step(), 1.1 * dt_prev)
+        L.advance(dt)
+        L.post_timestep()
+        t1 = time.perf_counter()
         del L
-    return {"value": rate["fast"], "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "value_1core": rate["fast_1core"], "value_parity_build": rate["parity"],
-            "build": orc.FAST_FLAGS, "parity_build": "-O2 -ffp-contract=off",
-            "host": {"cpu": _cpu_model(), "os_cpu_count": os.cpu_count()},
-            "sample": "%d^3 single-box Sedov (same problem constants), %d steps after one untimed step, "
-                      "oracle OpenMP over (1024,16,16) tiles; value_1core: one more step on one thread"
-                      % (n, steps)}
+        return zones / (t1 - t0)
+
+    out = {"value": timed_step(threads), "unit": "zone-updates/s", "cores": threads, "kind": "port",
+           "build": orc.FAST_FLAGS, "host": {"cpu": _cpu_model(), "os_cpu_count": os.cpu_count()},
+           "sample": "%s itself (%d^3, %d uneven boxes): one coarse step (estTimeStep, computeNewDt, "
+                     "advance, post_timestep) from the state the GPU run reached; oracle OpenMP over "
+                     "(1024,16,16) tiles" % (cfg["name"], n[0], len(cfg["boxes"]))}
+    if one_core:
+        out["value_1core"] = timed_step(1)
+    else:
+        rec = recorded_cpu_1core(cfg["name"])
+        if rec:
+            out["value_1core_recorded"] = rec
+    return out
+
+
+def recorded_cpu_1core(workload):
+    """One-thread rate of the same step from the committed summary of a
+    `bench.py --cpu-1core` run on the GPU box, or None."""
+    path = os.path.join(ROOT, "profiles", "cpu_baseline_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            w = ln.split()
+            if len(w) == 3 and w[0] == "cpu_1core_zone_updates_per_s" and w[1] == workload:
+                return {"value": float(w[2]), "source": os.path.relpath(path, ROOT)}
+    return None
+
+
+def batch_bench(cfg, steps=10, warmup=2):
+    """mc_advance_batch, the drop-in batched entry over caller-owned FABs, on
+    the benchmark workload: per call, gather every box's S_old FAB into the
+    engine, one fused step, scatter into the S_new FABs and return
+    estTimeStep (a host synchronisation per call, as the reference host
+    reads dt every step).  The FABs are the per-box state of a plain-mode
+    level (S_old) and one device tensor per box (S_new), swapped every call.
+    dt follows computeNewDt from the returned estimates."""
+    import torch
+    import minicastro as mc
+    n = cfg["n_cell"]
+    zones = int(n[0]) * int(n[1]) * int(n[2])
+    P = mc.Level(n, cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_PLAIN)
+    P.init_data(**cfg)
+    dt = P.initial_dt()
+    boxes = [tuple(int(v) for v in b) for b in cfg["boxes"]]
+    a = [int(C.cast(P.L.mc_level_state_ptr(P.h, ib), C.c_void_p).value) for ib in range(len(boxes))]
+    bt = [torch.empty(20 * (b[3] - b[0] + 1) * (b[4] - b[1] + 1) * (b[5] - b[2] + 1), dtype=torch.float64,
+                      device="cuda") for b in boxes]
+    b_ = [t.data_ptr() for t in bt]
+
+    def work(src, dst):
+        return [mc.work_item(b[:3], b[3:], src[ib], b[:3], b[3:], dst[ib], b[:3], b[3:])
+                for ib, b in enumerate(boxes)]
+    fwd, bwd = work(a, b_), work(b_, a)
+    k = 0
+    for _ in range(warmup):
+        est = mc.advance_batch(n, cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], fwd if k % 2 == 0 else bwd, dt)
+        dt = min(est, 1.1 * dt)
+        k += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        est = mc.advance_batch(n, cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], fwd if k % 2 == 0 else bwd, dt)
+        dt = min(est, 1.1 * dt)
+        k += 1
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mc.batch.release()
+    P.close()
+    del bt
+    return {"entry": "mc_advance_batch", "steps": steps, "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "zone_updates_per_s": zones * steps / (t1 - t0),
+            "per_call": "gather S_old FABs + one fused step + scatter S_new FABs + estTimeStep read back"}
 
 
 def ensemble_bench(steps=50, n_inst=24):
@@ -213,14 +280,16 @@ def ensemble_bench(steps=50, n_inst=24):
             "tiling": {k: list(v[0]) for k, v in tuned.items()}}
 
 
-def adjoint_bench(reps=3):
+def adjoint_bench(reps=3, ndir=16, n_tl=2):
     """Discrete adjoint on the 64^3 Sedov adjoint workload, 10 steps: time of
     the checkpointing forward pass and of the adjoint pass (the reversed
-    stages, reading each step's intermediates as the forward pass kept them),
-    the same with every step recomputed from its checkpoint instead, and the
-    central-difference check of two smooth random directions (eps 1e-6 and
-    1e-7)."""
-    import numpy as np
+    stages, reading each step's intermediates as the forward pass kept
+    them), the same with every step recomputed from its checkpoint instead;
+    central differences at eps 1e-6 along `ndir` per-cell white-noise
+    directions (the 16 random directions of the workload) and `ndir` smooth
+    directions (random low-order cosine modes), each with a random E
+    component; and the relative difference from the CPU tangent-linear of
+    the oracle along `n_tl` of them (smooth and white noise alternating)."""
     import torch
     from minicastro import configs
     from adjoint_harness import SedovAdjointProblem, smooth_direction
@@ -253,21 +322,48 @@ def adjoint_bench(reps=3):
         fr, ar = timed()
         P.adj.keep_intermediates(True)
         rec = {"forward_ms": fr, "adjoint_ms": ar, "adjoint_over_forward": ar / fr}
-    rng = np.random.default_rng(7)
-    errs = []
-    for q in range(2):
-        v = P.rho0 * torch.from_numpy(smooth_direction(rng, 64)).to("cuda")
-        vE = P.E * rng.standard_normal()
+        P.gradient()  # the kept intermediates of the unperturbed forward pass again
+
+    def fd_err(v, vE, eps=1e-6):
         ad = float((g_rho * v).sum()) + g_E * vE
-        e = []
-        for eps in (1e-6, 1e-7):
-            fd = (P.J(P.rho0 + eps * v, P.E + eps * vE) - P.J(P.rho0 - eps * v, P.E - eps * vE)) / (2 * eps)
-            e.append(abs(fd - ad) / abs(ad))
-        errs.append(e)
+        fd = (P.J(P.rho0 + eps * v, P.E + eps * vE) - P.J(P.rho0 - eps * v, P.E - eps * vE)) / (2 * eps)
+        return abs(fd - ad) / abs(ad)
+
+    rng = np.random.default_rng(7)
+    n = cfg["n_cell"][0]
+    dirs = {"white_noise": [], "smooth": []}
+    for q in range(ndir):
+        dirs["white_noise"].append((cfg["dens_ambient"] * rng.standard_normal((n, n, n)), P.E * rng.standard_normal()))
+        dirs["smooth"].append((P.rho0.cpu().numpy() * smooth_direction(rng, n), P.E * rng.standard_normal()))
+    fd = {}
+    for kind, lst in dirs.items():
+        e = [fd_err(torch.from_numpy(v).to("cuda"), vE) for v, vE in lst]
+        fd[kind] = {"n": len(e), "below_1e-6": int(sum(x < 1e-6 for x in e)), "max": max(e),
+                    "median": float(np.median(e)), "rel_err": e}
+    fd["E_only"] = fd_err(torch.zeros_like(P.rho0), P.E)
+    tl = None
+    if n_tl > 0:
+        from tl_problem import OracleSedovProblem
+        O = OracleSedovProblem(cfg)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        rel = []
+        for q in range(n_tl):
+            v, vE = dirs["smooth" if q % 2 == 0 else "white_noise"][q // 2]
+            ad = float((g_rho * torch.from_numpy(v).to("cuda")).sum()) + g_E * vE
+            _, t, _ = O.tangent(np.ascontiguousarray(v.transpose(2, 1, 0)), vE, P.dts, nthreads=threads)
+            rel.append(abs(ad - t) / abs(t))
+        tl = {"directions": n_tl, "max_rel_diff": max(rel), "rel_diff": rel,
+              "what": "CPU tangent-linear of the oracle (forward-mode dual numbers through the restated "
+                      "reference arithmetic), smooth and white-noise directions alternating"}
     return {"workload": cfg["name"], "steps": cfg["steps"], "forward_ms": f, "adjoint_ms": a_,
             "adjoint_over_forward": a_ / f, "intermediates": "kept" if kept else "recomputed",
             "recompute_mode": rec, "J": J, "dJ_dE": g_E,
-            "fd_rel_err_smooth_dirs": {"eps_1e-6": [e[0] for e in errs], "eps_1e-7": [e[1] for e in errs]}}
+            "fd_eps": 1e-6, "fd_central_differences": fd, "vs_tangent_linear": tl,
+            "deviations_from_workload": {
+                "r_init": cfg["r_init"], "exp_energy": cfg["exp_energy"], "init_shrink": None,
+                "why": "the unit-cube Sedov blast (r_init 0.01, E 1e12) with the initial 0.01 dt shrink "
+                       "does not reach the objective's shell r in [0.2, 0.3] in 10 steps; dt_0 = "
+                       "cfl * estdt(U0) and the blast starts at r = 0.15 with E = 1e14"}}
 
 
 def main():
@@ -280,6 +376,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the mc_advance_batch timing")
+    ap.add_argument("--cpu-1core", action="store_true",
+                    help="also time the CPU baseline step on one thread (about three minutes at 256^3)")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="only time the adjoint workload and print its JSON (for profiling)")
     ap.add_argument("--tiling-json", default=None,
@@ -324,6 +423,7 @@ def main():
         tuned = None
     else:
         tuned = L.autotune(reps=2)   # restores the state it times on
+        log("autotuned")
     tiling = {L.STAGES[st]: list(L.get_tiling(st)) for st in range(1, 8)}
     L.run(a.warmup, use_graph=True)
     L.sync()
@@ -343,18 +443,53 @@ def main():
     g1 = time.perf_counter()
     graph_ms = (g1 - g0) * 1e3 / a.steps
     fused_steps_done = L.steps
+    log("fused path timed")
+    cpu_state = None if a.no_cpu_baseline else (L.state(), L.dt)
     L.close()
 
+    # Whole step against the reference algorithm's work per zone-update
+    # (op-count build of the CPU restatement, committed summary) and the
+    # summed measured HBM traffic of the stages of one step (rocprof
+    # counters, committed summary) against the zone-update's 320 B.
+    w_flop = reference_flop_per_zone_update(cfg["name"])
+    tr = [measured_traffic("mc::K_" + st, cfg["name"]) for st in STAGES_WITH_BYTES]
+    step_traffic = sum(tr) if all(tr) else None
+    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in STAGES_WITH_BYTES]
+    hw_step = sum(hw) if all(hw) else None
+    roofline = {
+        "bound": "fp64", "unit": "TFLOP/s", "peak": FP64_PEAK / 1e12,
+        "achieved": (value * w_flop / 1e12) if w_flop else None,
+        "frac": (value * w_flop / FP64_PEAK) if w_flop else None,
+        "traffic": step_traffic,
+        "scope": "whole coarse step (7 fused stage launches + dt control); achieved = value x "
+                 "flop_per_zone_update (reference algorithm, counted by the op-count oracle build)",
+        "flop_per_zone_update": w_flop,
+        "traffic_scope": "HBM bytes per step, summed over the 7 stages (rocprof FETCH_SIZE + WRITE_SIZE, "
+                         "calibrated on a plain copy)",
+        "traffic_per_zone_update_bytes": (step_traffic / zones) if step_traffic else None,
+        "traffic_over_320B_per_zone_update": (step_traffic / zones / B_ALG_PER_ZONE_UPDATE) if step_traffic else None,
+        "hbm": {"bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE,
+                "achieved_GBs": value * B_ALG_PER_ZONE_UPDATE / 1e9,
+                "frac": value * B_ALG_PER_ZONE_UPDATE / HBM_PEAK},
+        "issued_fp64": {"ops_per_zone_update": (hw_step / zones) if hw_step else None,
+                        "TFLOPs": (hw_step / (ms_step / 1e3) / 1e12) if hw_step else None,
+                        "frac": (hw_step / (ms_step / 1e3) / FP64_PEAK) if hw_step else None},
+    }
+
+    # the dominant stage (HIP events on the level's stream) against its own
+    # intermediate arrays' bytes -- not the zone-update unit
     dom = max((k for k in stages if k in STAGES_WITH_BYTES), key=lambda k: stages[k][0])
     tot_ms, nl = stages[dom]
     avg_s = tot_ms / nl / 1e3
     traffic_alg = _stage_bytes(dom, n)
     achieved = traffic_alg / avg_s
     traffic = measured_traffic("mc::K_" + dom, cfg["name"])
-    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "mc::K_" + dom,
-                "avg_launch_ms": tot_ms / nl, "algorithmic_bytes_per_launch": traffic_alg,
-                "traffic_GB_per_s": (traffic / avg_s / 1e9) if traffic else None}
+    roofline["dominant_stage"] = {
+        "kernel": "mc::K_" + dom, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "avg_launch_ms": tot_ms / nl,
+        "bytes": "the stage's own input and output arrays (every distinct element once per launch)",
+        "algorithmic_bytes_per_launch": traffic_alg,
+        "traffic_GB_per_s": (traffic / avg_s / 1e9) if traffic else None}
 
     # FP64 pipe load of the EOS-bound stages (clean_state + ctoprim, update + estdt)
     fp64 = {}
@@ -365,25 +500,6 @@ def main():
             fp64[st] = {"bound": "fp64 valu", "achieved": rate / 1e12, "peak": FP64_PEAK / 1e12,
                         "unit": "TFLOP/s", "frac": rate / FP64_PEAK, "ops_per_launch": ops}
     roofline["fp64_stages"] = fp64 or None
-
-    # whole step: work of the reference algorithm per zone-update at the
-    # measured rate, and the FP64 operations the fused stages issue
-    # (rocprof counters, summed over the stages of one step)
-    w_flop = reference_flop_per_zone_update(cfg["name"])
-    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in STAGES_WITH_BYTES]
-    hw_step = sum(hw) if all(hw) else None
-    roofline["step"] = {
-        "unit_work": "zone-update",
-        "reference_flop_per_zone_update": w_flop,
-        "fp64_achieved_TFLOPs": (value * w_flop / 1e12) if w_flop else None,
-        "fp64_frac": (value * w_flop / FP64_PEAK) if w_flop else None,
-        "issued_fp64_ops_per_zone_update": (hw_step / zones) if hw_step else None,
-        "issued_fp64_TFLOPs": (hw_step / (ms_step / 1e3) / 1e12) if hw_step else None,
-        "issued_fp64_frac": (hw_step / (ms_step / 1e3) / FP64_PEAK) if hw_step else None,
-        "bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE,
-        "hbm_achieved_GBs": value * B_ALG_PER_ZONE_UPDATE / 1e9,
-        "hbm_frac": value * B_ALG_PER_ZONE_UPDATE / HBM_PEAK,
-    }
 
     # per-box path (the reference's kernel order) on the same workload
     plain = None
@@ -399,9 +515,15 @@ def main():
         plain = (p1 - p0) * 1e3 / a.plain_steps
         P.close()
 
+    log("per-box path done")
+    batch = None if a.no_batch else batch_bench(cfg)
+    log("mc_advance_batch done")
     adjoint = None if a.no_adjoint else adjoint_bench()
+    log("adjoint done")
     ensemble = None if a.no_ensemble else ensemble_bench()
-    cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
+    log("ensemble done")
+    cpu = None if cpu_state is None else cpu_baseline(cfg, cpu_state[0], cpu_state[1], one_core=a.cpu_1core)
+    log("cpu baseline done")
     out = {"metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -412,6 +534,7 @@ def main():
                       "tiling": tiling, "tiling_source": "autotuner" if tuned else a.tiling_json,
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,
+                      "batch_entry": batch,
                       "stage_ms_per_step": {k: v[0] / a.steps for k, v in stages.items() if v[1]}},
            "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
            "ensemble": ensemble}

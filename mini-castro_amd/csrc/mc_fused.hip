@@ -935,6 +935,10 @@ int fused_ninst(FusedLevel *F) { return F->ninst; }
 // estdt of the current state of every instance into dt_est (no cleaning)
 int fused_estimate(FusedLevel *F, hipStream_t s)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     fill(F->ctx.dt_est, F->ninst, DBL_MAX, s);
     launch_stage(F, ST_EST, s);
     F->have_est = true;
@@ -961,6 +965,10 @@ static void enqueue_step(FusedLevel *F, double stop_time, hipStream_t s)
 // number of launches (the dt kernels are counted under FST_N).
 int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, double *stage_ms, int *launches)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     if (!F->have_est && fused_estimate(F, s)) return -1;
     static const int order[7] = {ST_PRIM, ST_FLAT, ST_TRACE, ST_FLUX1, ST_TRANS1, ST_FINAL, ST_UPDATE};
     static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
@@ -1002,12 +1010,19 @@ int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s
 
 int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool use_graph)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     if (!F->have_est && fused_estimate(F, s)) return -1;
     if (!use_graph) {
         for (int n = 0; n < nsteps; ++n) enqueue_step(F, stop_time, s);
         return check_hip(hipGetLastError(), "fused_steps");
     }
-    if (!F->gexec || F->graph_stop != stop_time) {
+    // a captured graph holds the EOS table pointer: capture again when the
+    // table was loaded anew since (eos_finalize + a new load may move it)
+    if (!F->gexec || F->graph_stop != stop_time || F->graph_tab != eos_table().node ||
+        F->graph_fail != eos_table().newton_fail) {
         if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
         if (F->graph) (void)hipGraphDestroy(F->graph);
         F->gexec = nullptr;
@@ -1019,6 +1034,8 @@ int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool
         if (check_hip(hipStreamEndCapture(s, &F->graph), "end capture")) return -1;
         if (check_hip(hipGraphInstantiate(&F->gexec, F->graph, nullptr, nullptr, 0), "graph instantiate")) return -1;
         F->graph_stop = stop_time;
+        F->graph_tab = eos_table().node;
+        F->graph_fail = eos_table().newton_fail;
     }
     for (int n = 0; n < nsteps; ++n)
         if (check_hip(hipGraphLaunch(F->gexec, s), "graph launch")) return -1;
@@ -1027,6 +1044,10 @@ int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool
 
 int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     fill(F->ctx.dt_cur, F->ninst, dt, s);
     fill(F->ctx.dt_est, F->ninst, DBL_MAX, s);
     launch_stage(F, ST_PRIM, s);
@@ -1040,10 +1061,16 @@ int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s)
     return check_hip(hipGetLastError(), "fused_advance_dt");
 }
 
-// The stages of a step up to the final fluxes with a given dt (the state is
-// not updated): recomputes every intermediate array of the step.
+// The stages of a step up to the final fluxes with a given dt: recomputes
+// every intermediate array of the step.  K_prim replaces the state by its
+// clean in place (the clean(S_old) a step starts from); no later stage of a
+// step is run, so the state is not advanced.
 int fused_recompute_dt(FusedLevel *F, double dt, hipStream_t s)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     fill(F->ctx.dt_cur, F->ninst, dt, s);
     launch_stage(F, ST_PRIM, s);
     launch_stage(F, ST_FLAT, s);
@@ -1102,6 +1129,10 @@ void fused_get_tiling(FusedLevel *F, int stage, int *bxyz)
 
 int fused_run_stage(FusedLevel *F, int stage, hipStream_t s)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
     launch_stage(F, stage, s, ny[stage]);
     return check_hip(hipGetLastError(), "fused_run_stage");
@@ -1115,25 +1146,36 @@ static const KernelTiling tune_cand[] = {
     {16, 16, 1}, {16, 8, 2}, {16, 4, 4}, {8, 8, 4}, {128, 2, 1}};
 
 // Tiling autotuner: times every candidate brick of every stage of the step
-// on the data one coarse step leaves behind and keeps the fastest.  Apart
-// from the state, which K_prim (its clean) and K_update rewrite in place, no
-// stage reads an array it writes, so repeating a launch recomputes the same
-// outputs; the state and the clocks are restored afterwards, so results do
-// not change (tilings never change results).  bricks (8 x 3) and best_ms (8)
+// on the data one coarse step leaves behind and keeps the fastest.  The
+// stages between K_prim and K_update read no array they write, so repeating
+// one of their launches recomputes the same outputs.  K_prim (its clean) and
+// K_update rewrite the state in place, so each of their timed launches starts
+// from the state the step started from (K_update's from K_prim's clean of
+// it, as in a step), restored by untimed copies.  The state, the clocks and
+// the EOS Newton-failure counter are restored afterwards, so results do not
+// change (tilings never change results).  bricks (8 x 3) and best_ms (8)
 // receive the chosen brick and its time per launch; either may be null.
 int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *best_ms)
 {
+    if (!eos_ready()) {
+        set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
     if (reps < 1) reps = 1;
     if (!F->have_est && fused_estimate(F, s)) return -1;
     const int ni = F->ninst;
     double *saveU = nullptr, *saveC = nullptr;
     int *saveS = nullptr;
+    unsigned long long *saveN = nullptr;
+    unsigned long long *cnt = eos_table().newton_fail;
     const size_t nU = sizeof(double) * NVAR * (size_t)F->totD;
     if (check_hip(hipMalloc(&saveU, nU), "hipMalloc(autotune)") ||
         check_hip(hipMalloc(&saveC, sizeof(double) * 3 * ni), "hipMalloc(autotune)") ||
-        check_hip(hipMalloc(&saveS, sizeof(int) * ni), "hipMalloc(autotune)")) {
+        check_hip(hipMalloc(&saveS, sizeof(int) * ni), "hipMalloc(autotune)") ||
+        check_hip(hipMalloc(&saveN, sizeof(unsigned long long)), "hipMalloc(autotune)")) {
         (void)hipFree(saveU);
         (void)hipFree(saveC);
+        (void)hipFree(saveS);
         return -1;
     }
     Ctx &x = F->ctx;
@@ -1142,23 +1184,45 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
              check_hip(hipMemcpyAsync(saveC + ni, x.dt_est, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "save") ||
              check_hip(hipMemcpyAsync(saveC + 2 * ni, x.time, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "save") ||
              check_hip(hipMemcpyAsync(saveS, x.steps, sizeof(int) * ni, hipMemcpyDeviceToDevice, s), "save");
+    if (!rc && cnt)
+        rc = check_hip(hipMemcpyAsync(saveN, cnt, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s), "save");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) rc = check_hip(hipEventCreate(&e0), "hipEventCreate") || check_hip(hipEventCreate(&e1), "hipEventCreate");
     if (!rc) enqueue_step(F, -1.0, s);  // every intermediate array now holds data of a real step
     static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    auto restore_state = [&]() {
+        return check_hip(hipMemcpyAsync(x.U, saveU, nU, hipMemcpyDeviceToDevice, s), "autotune restore");
+    };
     for (int st = ST_PRIM; st <= ST_UPDATE && !rc; ++st) {
+        const bool writes_state = st == ST_PRIM || st == ST_UPDATE;
         KernelTiling best = F->tiling[st];
         double tbest = 1e300;
         for (const KernelTiling &c : tune_cand) {
             F->tiling[st] = c;
-            launch_stage(F, st, s, ny[st]);  // creates the block table outside the timed region
-            (void)hipEventRecord(e0, s);
-            for (int r = 0; r < reps; ++r) launch_stage(F, st, s, ny[st]);
-            (void)hipEventRecord(e1, s);
             float ms = 0.f;
-            if ((rc = check_hip(hipEventSynchronize(e1), "autotune sync") ||
-                      check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")))
-                break;
+            if (!writes_state) {
+                launch_stage(F, st, s, ny[st]);  // creates the block table outside the timed region
+                (void)hipEventRecord(e0, s);
+                for (int r = 0; r < reps; ++r) launch_stage(F, st, s, ny[st]);
+                (void)hipEventRecord(e1, s);
+                if ((rc = check_hip(hipEventSynchronize(e1), "autotune sync") ||
+                          check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")))
+                    break;
+            } else {
+                (void)get_plan(F, st);  // the block table, outside the timed region
+                for (int r = 0; r < reps && !rc; ++r) {
+                    if ((rc = restore_state())) break;
+                    if (st == ST_UPDATE) launch_stage(F, ST_PRIM, s);  // clean(S_old), as in a step
+                    (void)hipEventRecord(e0, s);
+                    launch_stage(F, st, s, ny[st]);
+                    (void)hipEventRecord(e1, s);
+                    float one = 0.f;
+                    rc = check_hip(hipEventSynchronize(e1), "autotune sync") ||
+                         check_hip(hipEventElapsedTime(&one, e0, e1), "hipEventElapsedTime");
+                    ms += one;
+                }
+                if (rc) break;
+            }
             if (ms / reps < tbest) {
                 tbest = ms / reps;
                 best = c;
@@ -1178,18 +1242,21 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
         bricks[2] = F->tiling[ST_EST].bz;
     }
     if (best_ms) best_ms[ST_EST] = 0.0;
-    // restore the state and the clocks
-    rc = check_hip(hipMemcpyAsync(x.U, saveU, nU, hipMemcpyDeviceToDevice, s), "restore") ||
+    // restore the state, the clocks and the Newton-failure counter
+    rc = restore_state() ||
          check_hip(hipMemcpyAsync(x.dt_cur, saveC, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
          check_hip(hipMemcpyAsync(x.dt_est, saveC + ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
          check_hip(hipMemcpyAsync(x.time, saveC + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
          check_hip(hipMemcpyAsync(x.steps, saveS, sizeof(int) * ni, hipMemcpyDeviceToDevice, s), "restore") ||
+         (cnt && check_hip(hipMemcpyAsync(cnt, saveN, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s),
+                           "restore")) ||
          check_hip(hipStreamSynchronize(s), "autotune restore") || rc;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(saveU);
     (void)hipFree(saveC);
     (void)hipFree(saveS);
+    (void)hipFree(saveN);
     if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
     if (F->graph) (void)hipGraphDestroy(F->graph);
     F->gexec = nullptr;

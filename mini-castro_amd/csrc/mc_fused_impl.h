@@ -351,6 +351,8 @@ struct FusedLevel {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double graph_stop = -2.0;
+    const double *graph_tab = nullptr;  // EOS table the captured graph reads
+    const unsigned long long *graph_fail = nullptr;  // and its Newton-failure counter
     // single-level import/export
     ArenaBox *abox = nullptr;
     long *abstart = nullptr;

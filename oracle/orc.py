@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+# ORC_LIB names another build of the same sources (the pin tests are run
+# against deliberately altered builds to show that they catch the change)
+LIB_PATH = os.environ.get("ORC_LIB") or os.path.join(HERE, "build", "liboracle.so")
 
 NVAR, QVAR, NQAUX, NGDNV, NSPEC = 20, 21, 4, 6, 13
 BC_PERIODIC, BC_OUTFLOW = 0, 1

@@ -96,3 +96,35 @@ def test_oversized_instance_rejected_before_allocation():
     with pytest.raises(RuntimeError, match="2\\^28"):
         mc.Ensemble([dict(n_cell=(812, 812, 812), prob_lo=(0, 0, 0), prob_hi=(1, 1, 1), bc=1,
                           p_ambient=1e13, dens_ambient=1.0, exp_energy=1e12, r_init=0.1)])
+
+
+def test_runs_refuse_without_table_and_recapture_after_reload():
+    """After eos_finalize every fused entry point reports an error instead of
+    launching with no table; after a new load a level that had captured its
+    step in a hipGraph captures it again (the table may have moved) and
+    still equals a level that never lost its table."""
+    cfg = small_cfg(n=16, bc=1, r_init=0.12)
+    A = mc.Level(cfg["n_cell"], cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_FUSED)
+    B = mc.Level(cfg["n_cell"], cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_FUSED)
+    A.init_data(**cfg)
+    B.init_data(**cfg)
+    A.run(2, use_graph=True)
+    B.run(2, use_graph=True)
+    L = mc.lib()
+    try:
+        L.eos_finalize()
+        for what, call in (("run", lambda: L.mc_level_run(A.h, 1, -1.0, 1)),
+                           ("run eager", lambda: L.mc_level_run(A.h, 1, -1.0, 0)),
+                           ("run_stage", lambda: L.mc_level_run_stage(A.h, 1)),
+                           ("autotune", lambda: L.mc_level_autotune(A.h, 1, None, None))):
+            assert call() != 0, what
+            assert b"EOS table" in L.mc_last_error(), what
+    finally:
+        mc.load_table(table_bin())
+    assert A.steps == 2
+    A.run(3, use_graph=True)
+    B.run(3, use_graph=True)
+    assert A.dt == B.dt and A.time == B.time
+    np.testing.assert_array_equal(A.state(), B.state())
+    A.close()
+    B.close()

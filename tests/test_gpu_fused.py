@@ -153,18 +153,34 @@ def test_config4_ensemble_24_instances_50_steps():
     """Config 4: the 24 seeded Sedov instances (sides 32-96, energies, radii
     and densities drawn per instance), 50 steps each with its own dt, all in
     one launch per stage.  Every instance's clock and state equal the same
-    instance advanced alone by the per-box path, bit for bit."""
+    instance advanced alone by the per-box path, bit for bit; every 32^3 and
+    48^3 instance (eight of them) also equals the oracle advanced alone with
+    its own problem constants, bit for bit over all 50 steps."""
+    import os
     insts = configs.ensemble(24, seed=7)
     ens = mc.Ensemble(insts)
     ens.init_data()
     ens.run(50, use_graph=True)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    checked = 0
     for m, d in enumerate(insts):
         L = gpu_level(d, mode=mc.MODE_PLAIN)
         L.run(50)
         dt, t, steps = ens.clock(m)
         assert steps == L.steps == 50 and dt == L.dt and t == L.time, (m, dt, L.dt)
-        np.testing.assert_array_equal(ens.state(m), L.state(), err_msg="instance %d" % m)
+        S = ens.state(m)
+        np.testing.assert_array_equal(S, L.state(), err_msg="instance %d" % m)
         L.close()
+        if d["n_cell"][0] in (32, 48):
+            O = oracle_level(d, nthreads=threads)
+            for _ in range(50):
+                O.coarse_step()
+            assert (O.steps, O.dt, O.time) == (steps, dt, t), (m, O.dt, dt)
+            np.testing.assert_array_equal(S, O.state(), err_msg="instance %d vs oracle" % m)
+            checked += 1
+            print("instance %d (%d^3, E %.3e, r %.4f, rho %.3f): oracle == ensemble over 50 steps" %
+                  (m, d["n_cell"][0], d["exp_energy"], d["r_init"], d["dens_ambient"]), flush=True)
+    assert checked == 8
     ens.close()
 
 

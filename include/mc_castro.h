@@ -166,6 +166,24 @@ void calculate_blast_radius(const int *lo, const int *hi,
 
 /* ===================== Part 2: extensions ===================== */
 
+/* Allocation-free calculate_blast_radius.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Same arguments, arithmetic and results (bit for bit) as
+   calculate_blast_radius above, but the block partial sums live in a
+   device array of fixed size that the library holds on every GPU it runs on
+   until the process ends, so the call allocates nothing.  A box may hold at
+   most MC_NOALLOC_MAX_CELLS cells; a larger one is refused (-1, see
+   mc_last_error) before anything is launched.  Every call on one GPU shares
+   that GPU's array: calls that overlap on one GPU (several host threads or
+   streams at once) race and must be serialised by the caller.  Returns 0 on
+   success; synchronises the stream. */
+#define MC_NOALLOC_MAX_CELLS 4194304
+int mc_calculate_blast_radius_noalloc(const int *lo, const int *hi,
+                                      double *state, const int *state_lo, const int *state_hi,
+                                      const double *dx, const double *problo, const double *probhi,
+                                      double *blast_mass, double *blast_radius, const double max_density);
+
 /* Stream for subsequent Part 1 launches of the calling thread (hipStream_t). */
 void mc_set_stream(void *stream);
 void *mc_get_stream(void);

@@ -21,6 +21,7 @@
 #include "mc_hydro.h"
 #include "mc_cell.h"
 #include "mc_internal.h"
+#include "../../include/mc_castro.h"
 
 namespace mc {
 
@@ -1028,3 +1029,86 @@ void calculate_blast_radius(const int *lo, const int *hi, double *state, const i
 }
 
 } // extern "C"
+
+// ---- allocation-free blast radius ----
+// The block partials of calculate_blast_radius and their two sums live in
+// `scratch`, a device array the library owns for the life of the process
+// (one copy per GPU), instead of memory allocated on the first call.
+namespace mc {
+
+constexpr long MC_NOALLOC_MAX_BLOCKS = MC_NOALLOC_MAX_CELLS / kBlock;
+static_assert(MC_NOALLOC_MAX_CELLS % kBlock == 0, "whole blocks");
+// 256 KiB + 16 B per GPU
+static __device__ __align__(256) double scratch[2 * MC_NOALLOC_MAX_BLOCKS + 2];
+
+// Sums the nb block partials in block order, as the host loop of
+// calculate_blast_radius does, so both entry points return the same bits.
+__global__ void k_blast_sum(const double *part, unsigned nb, double *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double m = 0, r = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+        m += part[2 * b];
+        r += part[2 * b + 1];
+    }
+    out[0] = m;
+    out[1] = r;
+}
+
+// Device address of `scratch` per device, taken once per device with that
+// device current (hipGetSymbolAddress); host code uses only these.
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64];
+
+static double *scratch_ptr()
+{
+    int dev = 0;
+    if (check_hip(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= 64) {
+        set_error("mc_calculate_blast_radius_noalloc: device index out of range");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        if (check_hip(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+            return nullptr;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+} // namespace mc
+
+extern "C" int mc_calculate_blast_radius_noalloc(const int *lo, const int *hi, double *state, const int *state_lo,
+                                                 const int *state_hi, const double *dx, const double *problo,
+                                                 const double *probhi, double *blast_mass, double *blast_radius,
+                                                 const double max_density)
+{
+    BoxIter it = make_iter(make_box(lo, hi));
+    const unsigned nb = grid_for(it.n);
+    if (nb == 0) return 0;
+    if (nb > MC_NOALLOC_MAX_BLOCKS) {
+        char buf[200];
+        snprintf(buf, sizeof(buf),
+                 "mc_calculate_blast_radius_noalloc: box of %ld cells exceeds the limit of %ld cells", (long)it.n,
+                 (long)MC_NOALLOC_MAX_BLOCKS * kBlock);
+        set_error(buf);
+        return -1;
+    }
+    double *part = scratch_ptr();
+    if (!part) return -1;
+    double *sums = part + 2 * (size_t)MC_NOALLOC_MAX_BLOCKS;
+    double c[3];
+    for (int d = 0; d < 3; ++d) c[d] = (probhi[d] - problo[d]) / 2.0;
+    // every partial the sum reads is written by this launch first: nothing to clear
+    hipLaunchKernelGGL(k_blast_radius, dim3(nb), dim3(kBlock), 0, t_stream, it, ARR(state), dx[0], dx[1], dx[2],
+                       problo[0], problo[1], problo[2], c[0], c[1], c[2], max_density, part);
+    hipLaunchKernelGGL(k_blast_sum, dim3(1), dim3(64), 0, t_stream, part, nb, sums);
+    double h[2] = {0.0, 0.0};
+    if (check_hip(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, t_stream), "blast d2h")) return -1;
+    if (check_hip(hipStreamSynchronize(t_stream), "blast sync")) return -1;
+    *blast_mass += h[0];
+    *blast_radius += h[1];
+    return 0;
+}

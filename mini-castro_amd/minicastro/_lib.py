@@ -61,6 +61,7 @@ SIGS = [
     ("initdata", None, [_ip, _ip] + FAB + [_dp, _dp, _dp]),
     ("denerror", None, [_ip, _ip, C.c_void_p, _ip, _ip] + FAB + [C.c_int8, C.c_int8]),
     ("calculate_blast_radius", None, [_ip, _ip] + FAB + [_dp, _dp, _dp, _dp, _dp, _D]),
+    ("mc_calculate_blast_radius_noalloc", _I, [_ip, _ip] + FAB + [_dp, _dp, _dp, _dp, _dp, _D]),
     ("mc_set_stream", None, [_vp]),
     ("mc_get_stream", _vp, []),
     ("mc_last_error", C.c_char_p, []),

@@ -65,3 +65,19 @@ def test_bad_box_lists_rejected():
     assert not mk([0, 0, 0, 7, 7, 3, 0, 0, 0, 7, 7, 3])  # overlap, volumes add up to the domain
     assert b"tile" in L.mc_last_error() or b"domain" in L.mc_last_error() or b"overlap" in L.mc_last_error()
 
+
+
+def test_noalloc_blast_radius_refuses_an_oversized_box_before_any_launch():
+    """MC_NOALLOC_MAX_CELLS cells at most; the refusal comes before any
+    device call, so it runs without a GPU."""
+    L = mc.lib()
+    src = open(HDR).read()
+    cap = int(re.search(r"#define MC_NOALLOC_MAX_CELLS (\d+)", src).group(1))
+    m, r = C.c_double(0.0), C.c_double(0.0)
+    iv = lambda v: (C.c_int * 3)(*v)
+    dv = lambda v: (C.c_double * 3)(*v)
+    hi = (cap // (128 * 128), 127, 127)   # one plane more than the cap
+    rc = L.mc_calculate_blast_radius_noalloc(iv((0, 0, 0)), iv(hi), None, iv((0, 0, 0)), iv(hi), dv((1, 1, 1)),
+                                             dv((0, 0, 0)), dv((1, 1, 1)), C.byref(m), C.byref(r), 1.0)
+    assert rc == -1 and b"exceeds the limit" in L.mc_last_error()
+    assert m.value == 0.0 and r.value == 0.0

@@ -157,6 +157,7 @@ def cpu_baseline(cfg, S, dt_prev, one_core=False):
     lib.orc_eos_init_arrays(orc.ptr(np.ascontiguousarray(table_array(), dtype=np.float64)))
 
     def timed_step(nthreads):
+        log("cpu baseline: one step on %d thread(s)" % nthreads)
         L = orc.Level(n, cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], cfg["boxes"], nthreads, _library=lib)
         L.set_state(S)
         t0 = time.perf_counter()

@@ -275,6 +275,56 @@ int mc_level_run_stage(mc_level_t *L, int stage);
    tuned); either may be NULL. */
 int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms);
 
+/* ---- AMR coupling of a coarse and a fine level ----
+   The flux register of a fine level (AMReX FluxRegister as the reference
+   uses it: defined at Source/Castro.cpp:69, CrseInit / FineAdd at
+   Source/Castro_advance.cpp:82-88, ClearInternalBorders + Reflux in
+   Castro::reflux, Source/Castro.cpp:451-486) and the average-down of
+   Castro::avgDown (Source/Castro.cpp:488-505, amrex::average_down).
+   fine_boxes: nfine x 6 ints (lo, hi) in the fine index space, each
+   coarsenable by ratio and inside the refined domain crse_n_cell * ratio.
+   Fluxes are what store_flux writes (dt * F * face area), face-centred FABs
+   in direction dir (0, 1, 2), NVAR components, Fortran order.  All pointers
+   are device pointers; calls run on the stream of mc_set_stream. */
+typedef struct mc_flux_register mc_flux_register_t;
+mc_flux_register_t *mc_flux_register_create(int nfine, const int *fine_boxes, const int *ratio,
+                                            const int *crse_n_cell, int bc);
+void mc_flux_register_destroy(mc_flux_register_t *R);
+/* doubles held: per fine box, direction and side (low, high) in that order,
+   the coarse faces of the coarsened box on that side x NVAR, Fortran order */
+long mc_flux_register_size(const mc_flux_register_t *R);
+int mc_flux_register_get(mc_flux_register_t *R, double *host);
+int mc_flux_register_set_val(mc_flux_register_t *R, double v);
+/* register := mult * flux on every register face the coarse flux FAB holds */
+int mc_flux_register_crse_init(mc_flux_register_t *R, const double *flux, const int *flux_lo, const int *flux_hi,
+                               int dir, double mult);
+/* register += mult * flux summed over the ratio^2 fine faces of each coarse
+   face, on both dir-faces of fine box fine_box (its flux FAB must hold them) */
+int mc_flux_register_fine_add(mc_flux_register_t *R, int fine_box, const double *flux, const int *flux_lo,
+                              const int *flux_hi, int dir, double mult);
+/* zero the faces where two fine boxes meet (periodic images included) */
+int mc_flux_register_clear_internal_borders(mc_flux_register_t *R);
+/* state(c) += -scale * reg / vol(c) for the coarse cell c just below a low
+   face and += scale * reg / vol(c) just above a high face, for the cells c
+   of box (lo, hi); faces in the order x, y, z low, then x, y, z high */
+int mc_flux_register_reflux(mc_flux_register_t *R, double *state, const int *state_lo, const int *state_hi,
+                            const int *lo, const int *hi, const double *vol, const int *vol_lo, const int *vol_hi,
+                            double scale);
+/* crse(c) = (sum of the ratio^3 fine cells of c) / ratio^3 for the coarse
+   cells c of box (lo, hi), ncomp components */
+int mc_average_down(const double *fine, const int *fine_lo, const int *fine_hi, double *crse, const int *crse_lo,
+                    const int *crse_hi, const int *lo, const int *hi, const int *ratio, int ncomp);
+/* Per-box path levels: crse_side is the register of the next finer level,
+   which this level's steps initialise with their store_flux output (mult
+   -1); fine_side is this level's own register, to which the steps of its
+   boxes that are fine boxes of the register add their fluxes (mult +1).
+   NULL detaches. */
+int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc_flux_register_t *fine_side);
+/* Castro::reflux into this coarse level's state, then the register is zeroed */
+int mc_level_reflux(mc_level_t *L, mc_flux_register_t *R);
+/* Castro::avgDown from the given boxes of the fine level (all if nfine < 0) */
+int mc_level_average_down(mc_level_t *fine, mc_level_t *crse, const int *ratio, int nfine, const int *fine_box_ids);
+
 /* ---- batched advance over caller-owned FABs ----
    Castro::advance followed by post_timestep's clean_state
    (Source/Castro_advance.cpp:11-100, Source/Castro.cpp:343-369) for every box

@@ -114,7 +114,7 @@ struct BoxIter {
     int lo0, lo1, lo2, nx, ny;
     long n;
 };
-inline BoxIter make_iter(const Box &b)
+__host__ __device__ inline BoxIter make_iter(const Box &b)
 {
     BoxIter it;
     it.lo0 = b.lo[0];

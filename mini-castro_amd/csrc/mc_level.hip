@@ -24,6 +24,7 @@
 #include "mc_common.h"
 #include "mc_fused.h"
 #include "mc_internal.h"
+#include "mc_amr.h"
 
 namespace mc {
 
@@ -160,6 +161,11 @@ struct mc_level {
     double time = 0.0, dt = 0.0;
     FusedLevel *fused = nullptr;
     bool fused_ahead = false;  // the fused engine's state is newer than S_new
+    // AMR coupling (per-box path): the finer level's register this level
+    // initialises with its coarse fluxes, this level's own register it adds
+    // its fine fluxes to, and the register's fine box of each level box (-1)
+    mc_flux_register *reg_crse = nullptr, *reg_fine = nullptr;
+    std::vector<int> fine_index;
 };
 
 static double *dalloc(size_t n, bool &ok)
@@ -588,6 +594,14 @@ static void hydro_box(mc_level_t *L, int ib, double dt)
         store_flux(ebx[d].lo, ebx[d].hi, L->fluxes_out[d], ebx[d].lo, ebx[d].hi, L->flux[d], gebx[d].lo, gebx[d].hi,
                    L->area[d], areab[d].lo, areab[d].hi, dt);
     }
+    // the flux registers take this box's store_flux output before the next
+    // box reuses the buffer (Castro_advance.cpp:82-88 reads the level's
+    // fluxes after the advance: the same values)
+    for (int d = 0; d < 3; ++d) {
+        if (L->reg_crse) (void)flux_register_crse_init(L->reg_crse, L->fluxes_out[d], ebx[d], d, -1.0, L->stream);
+        if (L->reg_fine && L->fine_index[ib] >= 0)
+            (void)flux_register_fine_add(L->reg_fine, L->fine_index[ib], L->fluxes_out[d], ebx[d], d, 1.0, L->stream);
+    }
     fill_hydro_source(bx.lo, bx.hi, state, fb.lo, fb.hi, q, qbx.lo, qbx.hi, source, bx.lo, bx.hi, L->flux[0],
                       gebx[0].lo, gebx[0].hi, L->flux[1], gebx[1].lo, gebx[1].hi, L->flux[2], gebx[2].lo, gebx[2].hi,
                       L->qe[0], gebx[0].lo, gebx[0].hi, L->qe[1], gebx[1].lo, gebx[1].hi, L->qe[2], gebx[2].lo,
@@ -632,6 +646,10 @@ int mc_level_advance(mc_level_t *L, double dt)
         return -1;
     }
     if (L->mode == MC_MODE_FUSED) {
+        if (L->reg_crse || L->reg_fine) {
+            set_error("mc_level_advance: flux registers are filled by the per-box path (MC_MODE_PLAIN)");
+            return -1;
+        }
         // the fused update includes post_timestep's clean_state
         if (!L->fused_ahead && sync_to_fused(L)) return -1;
         return fused_advance_dt(L->fused, dt, L->stream);
@@ -764,6 +782,116 @@ int mc_level_blast_radius(mc_level_t *L, double *blast_mass, double *blast_radiu
 int mc_level_steps(const mc_level_t *L) { return L->steps; }
 double mc_level_time(const mc_level_t *L) { return L->time; }
 double mc_level_dt(const mc_level_t *L) { return L->dt; }
+
+// ---- AMR coupling between two levels (per-box path) ----
+int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc_flux_register_t *fine_side)
+{
+    if ((crse_side || fine_side) && L->mode != MC_MODE_PLAIN) {
+        set_error("mc_level_set_flux_registers: flux registers are filled by the per-box path (MC_MODE_PLAIN)");
+        return -1;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (crse_side && crse_side->crse_n[d] != L->n_cell[d]) {
+            set_error("mc_level_set_flux_registers: the register's coarse domain is not this level's");
+            return -1;
+        }
+        if (fine_side && fine_side->crse_n[d] * fine_side->ratio[d] != L->n_cell[d]) {
+            set_error("mc_level_set_flux_registers: the register's refined domain is not this level's");
+            return -1;
+        }
+    }
+    L->reg_crse = crse_side;
+    L->reg_fine = fine_side;
+    L->fine_index.assign(L->boxes.size(), -1);
+    if (fine_side) {
+        for (size_t b = 0; b < L->boxes.size(); ++b)
+            for (size_t j = 0; j < fine_side->fine.size(); ++j) {
+                bool same = true;
+                for (int d = 0; d < 3; ++d)
+                    same = same && L->boxes[b].lo[d] == fine_side->fine[j].lo[d] &&
+                           L->boxes[b].hi[d] == fine_side->fine[j].hi[d];
+                if (same) L->fine_index[b] = (int)j;
+            }
+        for (size_t j = 0; j < fine_side->fine.size(); ++j)
+            if (std::find(L->fine_index.begin(), L->fine_index.end(), (int)j) == L->fine_index.end()) {
+                set_error("mc_level_set_flux_registers: a fine box of the register is not a box of this level");
+                L->reg_crse = L->reg_fine = nullptr;
+                return -1;
+            }
+    }
+    return 0;
+}
+
+// Castro::reflux (Source/Castro.cpp:451-486) into this (coarse) level's S_new
+int mc_level_reflux(mc_level_t *L, mc_flux_register_t *R)
+{
+    if (L->mode != MC_MODE_PLAIN) {
+        set_error("mc_level_reflux: per-box path (MC_MODE_PLAIN) only");
+        return -1;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (R->crse_n[d] != L->n_cell[d]) {
+            set_error("mc_level_reflux: the register's coarse domain is not this level's");
+            return -1;
+        }
+    if (alloc_plain(L)) return -1;
+    if (flux_register_clear_internal(R, L->stream)) return -1;
+    for (size_t b = 0; b < L->boxes.size(); ++b)
+        if (flux_register_reflux(R, L->S_new + L->off_valid[b], L->boxes[b], L->boxes[b], L->vol,
+                                 grow1(L->boxes[b], 4), 1.0, L->stream))
+            return -1;
+    if (flux_register_fill(R, 0.0, L->stream)) return -1;
+    return check_hip(hipStreamSynchronize(L->stream), "reflux");
+}
+
+// Castro::avgDown (Source/Castro.cpp:488-505): the coarse cells under the
+// given fine boxes (all of them when nfine < 0) become the mean of their
+// fine cells
+int mc_level_average_down(mc_level_t *fine, mc_level_t *crse, const int *ratio, int nfine, const int *fine_box_ids)
+{
+    if (fine->mode != MC_MODE_PLAIN || crse->mode != MC_MODE_PLAIN) {
+        set_error("mc_level_average_down: per-box path (MC_MODE_PLAIN) only");
+        return -1;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (ratio[d] < 1 || crse->n_cell[d] * ratio[d] != fine->n_cell[d]) {
+            set_error("mc_level_average_down: the levels' domains do not differ by the ratio");
+            return -1;
+        }
+    const int n = nfine < 0 ? (int)fine->boxes.size() : nfine;
+    if (check_hip(hipStreamSynchronize(fine->stream), "average_down: fine level sync")) return -1;
+    for (int q = 0; q < n; ++q) {
+        const int fb = nfine < 0 ? q : fine_box_ids[q];
+        if (fb < 0 || fb >= (int)fine->boxes.size()) {
+            set_error("mc_level_average_down: bad fine box index");
+            return -1;
+        }
+        const Box F = fine->boxes[fb];
+        Box C;
+        for (int d = 0; d < 3; ++d) {
+            if (F.lo[d] % ratio[d] || (F.hi[d] + 1) % ratio[d]) {
+                set_error("mc_level_average_down: a fine box is not coarsenable by the ratio");
+                return -1;
+            }
+            C.lo[d] = F.lo[d] / ratio[d];
+            C.hi[d] = (F.hi[d] + 1) / ratio[d] - 1;
+        }
+        for (size_t cb = 0; cb < crse->boxes.size(); ++cb) {
+            Box is;
+            bool any = true;
+            for (int d = 0; d < 3; ++d) {
+                is.lo[d] = std::max(C.lo[d], crse->boxes[cb].lo[d]);
+                is.hi[d] = std::min(C.hi[d], crse->boxes[cb].hi[d]);
+                any = any && is.lo[d] <= is.hi[d];
+            }
+            if (!any) continue;
+            if (average_down(fine->S_new + fine->off_valid[fb], F, crse->S_new + crse->off_valid[cb],
+                             crse->boxes[cb], is, ratio, NVAR, crse->stream))
+                return -1;
+        }
+    }
+    return check_hip(hipStreamSynchronize(crse->stream), "average_down");
+}
 
 } // extern "C"
 

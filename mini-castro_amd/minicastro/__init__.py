@@ -10,4 +10,4 @@ from .level import Level, MODE_PLAIN, MODE_FUSED, BC_PERIODIC, BC_OUTFLOW  # noq
 from .ensemble import Ensemble  # noqa: F401
 from .adjoint import Adjoint  # noqa: F401
 from .batch import advance_batch, work_item  # noqa: F401
-from . import configs  # noqa: F401
+from . import amr, configs  # noqa: F401

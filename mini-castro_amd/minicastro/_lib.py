@@ -95,6 +95,19 @@ SIGS = [
     ("mc_level_autotune", _I, [_vp, _I, _ip, _dp]),
     ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
     ("mc_level_blast_radius", _I, [_vp, _dp, _dp, _dp]),
+    ("mc_flux_register_create", _vp, [_I, _ip, _ip, _ip, _I]),
+    ("mc_flux_register_destroy", None, [_vp]),
+    ("mc_flux_register_size", C.c_long, [_vp]),
+    ("mc_flux_register_get", _I, [_vp, _dp]),
+    ("mc_flux_register_set_val", _I, [_vp, _D]),
+    ("mc_flux_register_crse_init", _I, [_vp, _dp, _ip, _ip, _I, _D]),
+    ("mc_flux_register_fine_add", _I, [_vp, _I, _dp, _ip, _ip, _I, _D]),
+    ("mc_flux_register_clear_internal_borders", _I, [_vp]),
+    ("mc_flux_register_reflux", _I, [_vp, _dp, _ip, _ip, _ip, _ip, _dp, _ip, _ip, _D]),
+    ("mc_average_down", _I, [_dp, _ip, _ip, _dp, _ip, _ip, _ip, _ip, _ip, _I]),
+    ("mc_level_set_flux_registers", _I, [_vp, _vp, _vp]),
+    ("mc_level_reflux", _I, [_vp, _vp]),
+    ("mc_level_average_down", _I, [_vp, _vp, _ip, _I, _ip]),
     ("mc_advance_batch", _I, [_ip, _dp, _dp, _I, _I, C.POINTER(BatchBox), _D, _dp]),
     ("mc_advance_batch_release", None, []),
     ("mc_ensemble_create", _vp, [_I, _ip, _dp, _dp, _ip]),

@@ -177,6 +177,29 @@ real orc_level_dt(const orc_level_t *L);
 /* hydro source of the last advance, for debugging (box ibox) */
 real *orc_level_source(orc_level_t *L, int ibox);
 
+/* AMR coupling of a coarse and a fine level (oracle/orc_amr.c) */
+typedef struct orc_flux_register orc_flux_register_t;
+orc_flux_register_t *orc_flux_register_create(int nfine, const int *fine_boxes, const int *ratio,
+                                              const int *crse_n, int bc);
+void orc_flux_register_destroy(orc_flux_register_t *R);
+long orc_flux_register_size(const orc_flux_register_t *R);
+real *orc_flux_register_data(orc_flux_register_t *R);
+void orc_flux_register_set_val(orc_flux_register_t *R, real v);
+void orc_flux_register_crse_init(orc_flux_register_t *R, const real *flux, const int *flux_lo,
+                                 const int *flux_hi, int dir, real mult);
+void orc_flux_register_fine_add(orc_flux_register_t *R, int fine_box, const real *flux, const int *flux_lo,
+                                const int *flux_hi, int dir, real mult);
+void orc_flux_register_clear_internal_borders(orc_flux_register_t *R);
+void orc_flux_register_reflux(orc_flux_register_t *R, real *state, const int *state_lo, const int *state_hi,
+                              const int *lo, const int *hi, const real *vol, const int *vol_lo,
+                              const int *vol_hi, real scale);
+void orc_average_down(const real *fine, const int *fine_lo, const int *fine_hi, real *crse, const int *crse_lo,
+                      const int *crse_hi, const int *lo, const int *hi, const int *ratio, int ncomp);
+int orc_flux_register_find(const orc_flux_register_t *R, const int *lo, const int *hi);
+void orc_level_set_flux_registers(orc_level_t *L, orc_flux_register_t *crse_side, orc_flux_register_t *fine_side);
+void orc_level_reflux(orc_level_t *L, orc_flux_register_t *R);
+void orc_level_average_down(orc_level_t *fine, orc_level_t *crse, const int *ratio, int nfine, const int *ids);
+
 #ifdef __cplusplus
 }
 #endif

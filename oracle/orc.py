@@ -69,7 +69,94 @@ def _bind(path):
         L.orc_eos_init.argtypes = [C.c_char_p]
         L.orc_ppm_reconstruct.argtypes = [_dp, C.c_double, _dp, _dp]
         L.orc_ppm_int_profile.argtypes = [C.c_double] * 6 + [_dp, _dp]
+        _bind_amr(L)
     return L
+
+
+def _bind_amr(L):
+    vp, I, D = C.c_void_p, C.c_int, C.c_double
+    L.orc_flux_register_create.restype = vp
+    L.orc_flux_register_create.argtypes = [I, _ip, _ip, _ip, I]
+    L.orc_flux_register_destroy.argtypes = [vp]
+    L.orc_flux_register_size.restype = C.c_long
+    L.orc_flux_register_size.argtypes = [vp]
+    L.orc_flux_register_data.restype = _dp
+    L.orc_flux_register_data.argtypes = [vp]
+    L.orc_flux_register_set_val.argtypes = [vp, D]
+    L.orc_flux_register_crse_init.argtypes = [vp, _dp, _ip, _ip, I, D]
+    L.orc_flux_register_fine_add.argtypes = [vp, I, _dp, _ip, _ip, I, D]
+    L.orc_flux_register_clear_internal_borders.argtypes = [vp]
+    L.orc_flux_register_reflux.argtypes = [vp, _dp, _ip, _ip, _ip, _ip, _dp, _ip, _ip, D]
+    L.orc_average_down.argtypes = [_dp, _ip, _ip, _dp, _ip, _ip, _ip, _ip, _ip, I]
+    L.orc_level_set_flux_registers.argtypes = [vp, vp, vp]
+    L.orc_level_reflux.argtypes = [vp, vp]
+    L.orc_level_average_down.argtypes = [vp, vp, _ip, I, _ip]
+
+
+def register_faces(fine_boxes, ratio):
+    """(fine box, dir, side, face lo, face hi) of every register face, in the
+    register's storage order (per fine box, direction, side)."""
+    out = []
+    for b, fb in enumerate(np.asarray(fine_boxes).reshape(-1, 6)):
+        clo = [int(fb[d]) // ratio[d] for d in range(3)]
+        chi = [(int(fb[3 + d]) + 1) // ratio[d] - 1 for d in range(3)]
+        for d in range(3):
+            for side in (0, 1):
+                lo, hi = list(clo), list(chi)
+                lo[d] = hi[d] = chi[d] + 1 if side else clo[d]
+                out.append((b, d, side, tuple(lo), tuple(hi)))
+    return out
+
+
+class FluxRegister:
+    """The oracle's flux register (oracle/orc_amr.c)."""
+
+    def __init__(self, fine_boxes, ratio, crse_n, bc, _library=None):
+        self.L = _library or lib()
+        fb = np.ascontiguousarray(np.asarray(fine_boxes, dtype=np.int32).reshape(-1, 6))
+        self.fine_boxes, self.ratio = fb, tuple(int(r) for r in ratio)
+        self.h = self.L.orc_flux_register_create(len(fb), fb.ctypes.data_as(_ip), iv(*ratio), iv(*crse_n), int(bc))
+        self.faces = register_faces(fb, self.ratio)
+
+    def __del__(self):
+        try:
+            self.L.orc_flux_register_destroy(self.h)
+        except Exception:
+            pass
+
+    def data(self):
+        n = self.L.orc_flux_register_size(self.h)
+        return np.ctypeslib.as_array(self.L.orc_flux_register_data(self.h), shape=(n,))
+
+    def face_arrays(self):
+        """{(fine box, dir, side): (lo, hi, array of shape face + (NVAR,))}"""
+        d, off, out = self.data(), 0, {}
+        for b, dr, side, lo, hi in self.faces:
+            shp = tuple(h - l + 1 for l, h in zip(lo, hi))
+            n = int(np.prod(shp))
+            out[(b, dr, side)] = (lo, hi, d[off:off + n * NVAR].reshape(shp + (NVAR,), order="F"))
+            off += n * NVAR
+        return out
+
+    def set_val(self, v):
+        self.L.orc_flux_register_set_val(self.h, float(v))
+
+    def crse_init(self, fab, d, mult):
+        self.L.orc_flux_register_crse_init(self.h, *fab.args(), int(d), float(mult))
+
+    def fine_add(self, j, fab, d, mult):
+        self.L.orc_flux_register_fine_add(self.h, int(j), *fab.args(), int(d), float(mult))
+
+    def clear_internal_borders(self):
+        self.L.orc_flux_register_clear_internal_borders(self.h)
+
+    def reflux(self, state, lo, hi, vol, scale=1.0):
+        self.L.orc_flux_register_reflux(self.h, *state.args(), iv(*lo), iv(*hi), *vol.args(), float(scale))
+
+
+def average_down(fine, crse, lo, hi, ratio, ncomp=None):
+    lib().orc_average_down(*fine.args(), *crse.args(), iv(*lo), iv(*hi), iv(*ratio),
+                           int(ncomp if ncomp is not None else fine.nc))
 
 
 def lib():
@@ -220,6 +307,19 @@ class Level:
 
     def coarse_step(self, stop_time=-1.0):
         return self.L.orc_level_coarse_step(self.h, stop_time)
+
+    def set_flux_registers(self, crse_side=None, fine_side=None):
+        self._regs = (crse_side, fine_side)   # kept alive with the level
+        self.L.orc_level_set_flux_registers(self.h, crse_side.h if crse_side else None,
+                                            fine_side.h if fine_side else None)
+
+    def reflux(self, R):
+        self.L.orc_level_reflux(self.h, R.h)
+
+    def average_down_from(self, fine, ratio, fine_box_ids=None):
+        ids = None if fine_box_ids is None else np.ascontiguousarray(fine_box_ids, dtype=np.int32)
+        self.L.orc_level_average_down(fine.h, self.h, iv(*ratio), -1 if ids is None else len(ids),
+                                      None if ids is None else ids.ctypes.data_as(_ip))
 
     @property
     def steps(self):

@@ -46,6 +46,11 @@ struct orc_level {
     int steps;
     real time, dt;
     int tile[3];
+    /* AMR coupling: the level's fluxes MultiFab (store_flux output, one FAB
+       per box and direction, allocated when a register is attached), the
+       finer level's register this level initialises and its own register */
+    real **fluxes[3];
+    orc_flux_register_t *reg_crse, *reg_fine;
 };
 
 static box_t vbox(const orc_level_t *L, int b) { return L->box[b]; }
@@ -93,6 +98,11 @@ void orc_level_destroy(orc_level_t *L)
         free(L->S_new[b]); free(L->S_old[b]); free(L->Sborder[b]); free(L->src[b]);
     }
     free(L->S_new); free(L->S_old); free(L->Sborder); free(L->src);
+    for (int d = 0; d < 3; ++d)
+        if (L->fluxes[d]) {
+            for (int b = 0; b < L->nbox; ++b) free(L->fluxes[d][b]);
+            free(L->fluxes[d]);
+        }
     free(L->box); free(L->owner); free(L);
 }
 
@@ -291,8 +301,14 @@ static void hydro_tile(orc_level_t *L, int ib, box_t bx, real dt)
         orc_apply_av(ebx[d].lo, ebx[d].hi, d + 1, dx, div, obx.lo, obx.hi, state, fb.lo, fb.hi,
                      flux[d], gebx[d].lo, gebx[d].hi);
         orc_normalize_species_fluxes(ebx[d].lo, ebx[d].hi, flux[d], gebx[d].lo, gebx[d].hi);
-        orc_store_flux(ebx[d].lo, ebx[d].hi, fluxes_out[d], ebx[d].lo, ebx[d].hi, flux[d],
-                       gebx[d].lo, gebx[d].hi, area[d], areab[d].lo, areab[d].hi, dt);
+        if (L->fluxes[d]) { /* (*fluxes[d])[mfi], Source/Castro_hydro.cpp:380-386 */
+            box_t nb = nodes(L->box[ib], d);
+            orc_store_flux(ebx[d].lo, ebx[d].hi, L->fluxes[d][ib], nb.lo, nb.hi, flux[d],
+                           gebx[d].lo, gebx[d].hi, area[d], areab[d].lo, areab[d].hi, dt);
+        } else {
+            orc_store_flux(ebx[d].lo, ebx[d].hi, fluxes_out[d], ebx[d].lo, ebx[d].hi, flux[d],
+                           gebx[d].lo, gebx[d].hi, area[d], areab[d].lo, areab[d].hi, dt);
+        }
     }
     orc_fill_hydro_source(bx.lo, bx.hi, state, fb.lo, fb.hi, q, qbx.lo, qbx.hi, source, sb.lo, sb.hi,
                           flux[0], gebx[0].lo, gebx[0].hi, flux[1], gebx[1].lo, gebx[1].hi, flux[2],
@@ -337,6 +353,74 @@ void orc_level_advance(orc_level_t *L, real dt)
         for (long c = 0; c < n; ++c) s[c] = s[c] + dt * h[c];
     }
     clean_state(L, L->S_new, 0); /* :78 */
+    /* flux registers, :82-88 */
+    if (L->reg_crse)
+        for (int b = 0; b < L->nbox; ++b)
+            for (int d = 0; d < 3; ++d) {
+                box_t nb = nodes(L->box[b], d);
+                orc_flux_register_crse_init(L->reg_crse, L->fluxes[d][b], nb.lo, nb.hi, d, -1.0);
+            }
+    if (L->reg_fine)
+        for (int b = 0; b < L->nbox; ++b) {
+            int j = orc_flux_register_find(L->reg_fine, L->box[b].lo, L->box[b].hi);
+            if (j < 0) continue;
+            for (int d = 0; d < 3; ++d) {
+                box_t nb = nodes(L->box[b], d);
+                orc_flux_register_fine_add(L->reg_fine, j, L->fluxes[d][b], nb.lo, nb.hi, d, 1.0);
+            }
+        }
+}
+
+void orc_level_set_flux_registers(orc_level_t *L, orc_flux_register_t *crse_side, orc_flux_register_t *fine_side)
+{
+    L->reg_crse = crse_side;
+    L->reg_fine = fine_side;
+    if ((crse_side || fine_side) && !L->fluxes[0])
+        for (int d = 0; d < 3; ++d) {
+            L->fluxes[d] = (real **)calloc(L->nbox, sizeof(real *));
+            for (int b = 0; b < L->nbox; ++b) L->fluxes[d][b] = alloc_fab(nodes(L->box[b], d), ORC_NVAR);
+        }
+}
+
+/* Castro::reflux, Source/Castro.cpp:451-486 */
+void orc_level_reflux(orc_level_t *L, orc_flux_register_t *R)
+{
+    orc_flux_register_clear_internal_borders(R);
+    const real v = L->dx[0] * L->dx[1] * L->dx[2]; /* Geometry::GetVolume */
+    for (int b = 0; b < L->nbox; ++b) {
+        real *vol = const_fab(L->box[b], v);
+        orc_flux_register_reflux(R, L->S_new[b], L->box[b].lo, L->box[b].hi, L->box[b].lo, L->box[b].hi, vol,
+                                 L->box[b].lo, L->box[b].hi, 1.0);
+        free(vol);
+    }
+    orc_flux_register_set_val(R, 0.0);
+}
+
+/* Castro::avgDown, Source/Castro.cpp:488-505, from the given fine boxes
+   (all when nfine < 0) */
+void orc_level_average_down(orc_level_t *F, orc_level_t *Cl, const int *ratio, int nfine, const int *ids)
+{
+    int n = nfine < 0 ? F->nbox : nfine;
+    for (int q = 0; q < n; ++q) {
+        int fb = nfine < 0 ? q : ids[q];
+        box_t fbox = F->box[fb], cc;
+        for (int d = 0; d < 3; ++d) {
+            cc.lo[d] = fbox.lo[d] / ratio[d];
+            cc.hi[d] = (fbox.hi[d] + 1) / ratio[d] - 1;
+        }
+        for (int cb = 0; cb < Cl->nbox; ++cb) {
+            box_t is;
+            int any = 1;
+            for (int d = 0; d < 3; ++d) {
+                is.lo[d] = cc.lo[d] > Cl->box[cb].lo[d] ? cc.lo[d] : Cl->box[cb].lo[d];
+                is.hi[d] = cc.hi[d] < Cl->box[cb].hi[d] ? cc.hi[d] : Cl->box[cb].hi[d];
+                any = any && is.lo[d] <= is.hi[d];
+            }
+            if (!any) continue;
+            orc_average_down(F->S_new[fb], fbox.lo, fbox.hi, Cl->S_new[cb], Cl->box[cb].lo, Cl->box[cb].hi, is.lo,
+                             is.hi, ratio, ORC_NVAR);
+        }
+    }
 }
 
 void orc_level_post_timestep(orc_level_t *L) { clean_state(L, L->S_new, 0); } /* Castro.cpp:369 */

@@ -242,6 +242,35 @@ def batch_bench(cfg, steps=10, warmup=2):
             "per_call": "gather S_old FABs + one fused step + scatter S_new FABs + estTimeStep read back"}
 
 
+def large_domain_bench(n=512, steps=3):
+    """A domain whose whole-level face arrays would not fit one GPU (n^3
+    Sedov, one instance, outflow): the engine runs the face stages in
+    z-slabs chosen by free memory.  Steps are hipGraph replays."""
+    import ctypes as C
+    import torch
+    import minicastro as mc
+    from minicastro import configs
+    torch.cuda.empty_cache()
+    d = dict(configs.UNIT_SEDOV)
+    d.update(name="sedov%d_1box" % n, n_cell=(n, n, n), boxes=configs.grid_boxes(n, n))
+    E = mc.Ensemble([d])
+    slab = E.slab
+    E.init_data()
+    E.run(1, use_graph=True)      # capture; the first step's EOS solves start from the initial guess
+    t0 = time.perf_counter()
+    E.run(steps, use_graph=True)  # returns after the stream has drained
+    t1 = time.perf_counter()
+    dt, tm, st = E.clock(0)
+    E.close()
+    nc = (C.c_int * 3)(n, n, n)
+    used = mc.lib().mc_fused_bytes(1, nc, slab)
+    whole = mc.lib().mc_fused_bytes(1, nc, 0)
+    return {"workload": d["name"], "cells": n ** 3, "steps": steps, "slab_planes": slab,
+            "ms_per_step": (t1 - t0) * 1e3 / steps, "zone_updates_per_s": n ** 3 * steps / (t1 - t0),
+            "engine_GB": used / 1e9, "engine_bytes_per_cell": used / n ** 3,
+            "whole_level_engine_GB": whole / 1e9, "steps_advanced": st}
+
+
 def ensemble_bench(steps=50, n_inst=24):
     """24 independent Sedov instances (sides in {32,48,64,96}, seeded
     energies, radii, densities), every instance with its own dt, advanced
@@ -378,6 +407,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble run")
     ap.add_argument("--no-batch", action="store_true", help="skip the mc_advance_batch timing")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512^3 z-slab run")
     ap.add_argument("--cpu-1core", action="store_true",
                     help="also time the CPU baseline step on one thread (about three minutes at 256^3)")
     ap.add_argument("--only-adjoint", action="store_true",
@@ -517,6 +547,8 @@ def main():
         P.close()
 
     log("per-box path done")
+    large = None if a.no_large else large_domain_bench()
+    log("large domain done")
     batch = None if a.no_batch else batch_bench(cfg)
     log("mc_advance_batch done")
     adjoint = None if a.no_adjoint else adjoint_bench()
@@ -536,6 +568,11 @@ def main():
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,
                       "batch_entry": batch,
+                      "engine_bytes_per_cell": {
+                          "whole_level": mc.lib().mc_fused_bytes(1, (C.c_int * 3)(*n), 0) / zones,
+                          "slabs_of_32_planes": mc.lib().mc_fused_bytes(1, (C.c_int * 3)(*n), 32) / zones,
+                          "state": 20 * 8},
+                      "large_domain": large,
                       "stage_ms_per_step": {k: v[0] / a.steps for k, v in stages.items() if v[1]}},
            "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
            "ensemble": ensemble}

@@ -260,6 +260,17 @@ int mc_level_blast_radius(mc_level_t *L, double *blast_mass, double *blast_radiu
 double mc_level_time(const mc_level_t *L);
 double mc_level_dt(const mc_level_t *L);
 int mc_level_sync(mc_level_t *L);
+/* The fused engine keeps its face-stage arrays (traces, fluxes: about
+   3.1 KB per cell) for the whole level, or for one z-slab of `planes` cell
+   planes at a time, the face stages and the update then running slab after
+   slab (about 0.35 KB per cell plus 3.1 KB per cell of one slab and its
+   two ring planes).  Results are the same bit for bit.  planes = -1 (the
+   default) keeps the whole level when that takes at most half the free
+   device memory, else picks the deepest slab that does; 0 forces the whole
+   level.  Changing it rebuilds the engine (state and clock kept).
+   mc_level_fused_slab returns the planes in use (0: whole level). */
+int mc_level_set_fused_slab(mc_level_t *L, int planes);
+int mc_level_fused_slab(mc_level_t *L);
 /* Fused-mode tiling: stage 0..7 = est, prim, flat, trace, flux1, trans1,
    final, update; a brick of bx*by*bz threads (multiple of 64, <= 256). */
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
@@ -358,14 +369,24 @@ void mc_advance_batch_release(void);
    and time-step control; the fused engine advances all of them with one
    launch per stage.  Instance state: 20 components, Fortran order over the
    whole instance box (i fastest, component slowest).  The fused engine (this,
-   MC_MODE_FUSED levels, mc_advance_batch, the adjoint) takes domains with
-   fewer than 2^28 cells including one ghost layer, (nx+2)(ny+2)(nz+2) <
-   2^28, and refuses larger ones with an error (more than one GPU's memory
-   would hold in any case). */
+   MC_MODE_FUSED levels, mc_advance_batch, the adjoint) takes instances of
+   fewer than 2^29 cells; with its face arrays covering whole instances (no
+   z-slabs, and always for the adjoint) also fewer than 2^28 cells including
+   one ghost layer, (nx+2)(ny+2)(nz+2) < 2^28.  Larger ones are refused with
+   an error before any device memory is taken. */
 typedef struct mc_ensemble mc_ensemble_t;
 /* n_cell, prob_lo, prob_hi: 3 per instance; bc: one per instance */
 mc_ensemble_t *mc_ensemble_create(int ninst, const int *n_cell, const double *prob_lo, const double *prob_hi,
                                   const int *bc);
+/* the same with the z-slab planes of the engine (as mc_level_set_fused_slab;
+   the default of mc_ensemble_create is -1) */
+mc_ensemble_t *mc_ensemble_create_slabs(int ninst, const int *n_cell, const double *prob_lo, const double *prob_hi,
+                                        const int *bc, int planes);
+int mc_ensemble_slab(const mc_ensemble_t *E);
+/* device bytes the fused engine holds for these instances (state, cell
+   arrays, face arrays) with z-slabs of `planes` cell planes (0: whole
+   instances); a host computation, no device needed */
+long mc_fused_bytes(int ninst, const int *n_cell, int planes);
 void mc_ensemble_destroy(mc_ensemble_t *E);
 int mc_ensemble_ninst(const mc_ensemble_t *E);
 double *mc_ensemble_state_ptr(mc_ensemble_t *E, int m);

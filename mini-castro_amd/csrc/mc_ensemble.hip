@@ -24,6 +24,16 @@ extern "C" {
 mc_ensemble_t *mc_ensemble_create(int ninst, const int *n_cell, const double *prob_lo, const double *prob_hi,
                                   const int *bc)
 {
+    return mc_ensemble_create_slabs(ninst, n_cell, prob_lo, prob_hi, bc, -1);
+}
+
+mc_ensemble_t *mc_ensemble_create_slabs(int ninst, const int *n_cell, const double *prob_lo, const double *prob_hi,
+                                        const int *bc, int planes)
+{
+    if (planes < -1) {
+        set_error("mc_ensemble_create_slabs: planes must be -1 (by memory), 0 (whole instances) or positive");
+        return nullptr;
+    }
     if (ninst <= 0) {
         set_error("mc_ensemble_create: no instances");
         return nullptr;
@@ -51,7 +61,7 @@ mc_ensemble_t *mc_ensemble_create(int ninst, const int *n_cell, const double *pr
         E->spec.push_back(s);
     }
     if (check_hip(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "hipStreamCreate") ||
-        !(E->F = fused_create_set(E->spec))) {
+        !(E->F = fused_create_set(E->spec, planes < 0 ? fused_auto_slab(E->spec) : planes))) {
         mc_ensemble_destroy(E);
         return nullptr;
     }
@@ -68,6 +78,18 @@ void mc_ensemble_destroy(mc_ensemble_t *E)
 }
 
 int mc_ensemble_ninst(const mc_ensemble_t *E) { return E->ninst; }
+int mc_ensemble_slab(const mc_ensemble_t *E) { return fused_slab(E->F); }
+
+long mc_fused_bytes(int ninst, const int *n_cell, int planes)
+{
+    std::vector<FusedInstanceSpec> spec(ninst > 0 ? ninst : 0);
+    for (int m = 0; m < ninst; ++m)
+        for (int d = 0; d < 3; ++d) {
+            spec[m].n[d] = n_cell[3 * m + d];
+            spec[m].dx[d] = 1.0;
+        }
+    return (long)fused_bytes_for(spec, planes);
+}
 
 double *mc_ensemble_state_ptr(mc_ensemble_t *E, int m)
 {

@@ -18,9 +18,16 @@ struct FusedInstanceSpec {
 
 enum { FST_EST = 0, FST_PRIM, FST_FLAT, FST_TRACE, FST_FLUX1, FST_TRANS1, FST_FINAL, FST_UPDATE, FST_N };
 
-FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec);
+// slab > 0: the face stages run in z-slabs of that many cell planes, so the
+// face arrays hold one slab (and its ring) instead of whole instances; 0: whole
+// instances.  fused_auto_slab picks 0 when whole-instance arrays fit in the
+// free device memory with room to spare, else the deepest slab that fits.
+FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec, int slab = 0);
 // single level: the box list only describes the state arena for import/export
-FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int bc, const double *dx);
+FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int bc, const double *dx, int slab = 0);
+size_t fused_bytes_for(const std::vector<FusedInstanceSpec> &spec, int slab);  // device bytes of an engine
+int fused_auto_slab(const std::vector<FusedInstanceSpec> &spec);
+int fused_slab(const FusedLevel *F);
 void fused_destroy(FusedLevel *F);
 int fused_import(FusedLevel *F, const double *arena, hipStream_t s);
 int fused_export(FusedLevel *F, double *arena, hipStream_t s);

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     const int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     const bool do_p = pos >= 0;
-    const bool do_m = pos <= I.n[d] - 1;
+    // the minus state of face pos + 1; in z it must lie in the slab's faces
+    const bool do_m = pos <= I.n[d] - 1 && (d != 2 || k + 1 <= I.zlo + I.zn);
     const int cun = FQ_U + d, cut = d == 0 ? FQ_V : (d == 1 ? FQ_W : FQ_U), cutt = d == 0 ? FQ_W : (d == 1 ? FQ_U : FQ_V);
     const idx_t c0 = cidx(I, i, j, k);
     const double cc = CA(x.Q, I, NFQ, FQ_C, c0);
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K
     const int d = sub_;
     const FInst &I = p.inst[m];
     const int pos = d == 0 ? i : (d == 1 ? j : k);
-    if (pos < 0) return;
+    if (pos < lo_index(I, d)) return;
     const idx_t e = eidx(I, i, j, k);
     FaceH L, R;
     load_fs_h(x.SM[d], I, e, d, L);
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
     // range: t-face 0..n_t, d-cells 0..n_d-1, the third direction -1..n
-    if (c[t] < 0 || c[d] < 0 || c[d] > I.n[d] - 1) return;
+    if (c[t] < lo_index(I, t) || c[d] < lo_index(I, d) || c[d] > hi_cell(I, d)) return;
     const idx_t e = eidx(I, i, j, k);
     const int et[3] = {t == 0, t == 1, t == 2}, ed[3] = {d == 0, d == 1, d == 2};
     const double cdtdx = x.dt_cur[m] / I.dx[d];
@@ -378,8 +379,8 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
     for (int a = 0; a < 3; ++a) {
-        if (c[a] < 0) return;
-        if (a != d && c[a] > I.n[a] - 1) return;
+        if (c[a] < lo_index(I, a)) return;
+        if (a != d && c[a] > hi_cell(I, a)) return;
     }
     const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
     const int ed[3] = {d == 0, d == 1, d == 2}, e1[3] = {t1 == 0, t1 == 1, t1 == 2}, e2[3] = {t2 == 0, t2 == 1, t2 == 2};
@@ -609,24 +610,35 @@ __global__ __launch_bounds__(256) void K_arena(const ArenaBox *boxes, int nbox, 
 static const int stage_shift[NSTAGE] = {0, 0, 1, 1, 1, 1, 0, 0};
 static const int stage_add[NSTAGE] = {0, 0, 2, 2, 2, 2, 1, 0};
 
-Plan get_plan(FusedLevel *F, int stage)
+// the cell stages before and after the face stages of a step (estdt,
+// K_prim) run over whole instances; the face stages and K_update run slab
+// by slab
+static bool whole_instance_stage(int stage) { return stage == ST_EST || stage == ST_PRIM; }
+
+Plan get_plan(FusedLevel *F, int stage, int slab)
 {
+    if (whole_instance_stage(stage)) slab = -1;
     const KernelTiling t = F->tiling[stage];
     const int sh = stage_shift[stage], add = stage_add[stage];
+    const FInst *tab = slab < 0 ? F->inst : F->sinst[slab];
     for (auto &pb : F->plans)
-        if (pb.t.bx == t.bx && pb.t.by == t.by && pb.t.bz == t.bz && pb.shift == sh && pb.add == add)
-            return Plan{F->inst, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+        if (pb.t.bx == t.bx && pb.t.by == t.by && pb.t.bz == t.bz && pb.shift == sh && pb.add == add &&
+            pb.slab == slab)
+            return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
     FusedLevel::PlanBuf pb;
     pb.t = t;
     pb.shift = sh;
     pb.add = add;
+    pb.slab = slab;
+    const std::vector<FInst> &hs = slab < 0 ? F->hinst : F->hsinst[slab];
     std::vector<int> b0(F->ninst + 1);
     int acc = 0;
     for (int m = 0; m < F->ninst; ++m) {
         b0[m] = acc;
-        const FInst &I = F->hinst[m];
-        const int e0 = I.n[0] + add, e1 = I.n[1] + add, e2 = I.n[2] + add;
-        acc += ((e0 + t.bx - 1) / t.bx) * ((e1 + t.by - 1) / t.by) * ((e2 + t.bz - 1) / t.bz);
+        const FInst &I = hs[m];
+        const int e0 = I.n[0] + add, e1 = I.n[1] + add, e2 = I.zn + add;
+        // an instance with no cells in this slab launches no bricks
+        if (I.zn > 0) acc += ((e0 + t.bx - 1) / t.bx) * ((e1 + t.by - 1) / t.by) * ((e2 + t.bz - 1) / t.bz);
     }
     b0[F->ninst] = acc;
     pb.nblocks = acc;
@@ -634,7 +646,7 @@ Plan get_plan(FusedLevel *F, int stage)
         check_hip(hipMemcpy(pb.blk0, b0.data(), sizeof(int) * (F->ninst + 1), hipMemcpyHostToDevice), "plan h2d"))
         return Plan{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
     F->plans.push_back(pb);
-    return Plan{F->inst, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+    return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
 }
 
 int plan_blocks(FusedLevel *F, const Plan &p)
@@ -650,10 +662,10 @@ using namespace mc;
 
 namespace mc {
 
-static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1)
+static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1, int slab = 0)
 {
     F->ctx.tab = eos_table();
-    Plan p = get_plan(F, stage);
+    Plan p = get_plan(F, stage, slab);
     const int nb = plan_blocks(F, p);
     const int nt = p.bx * p.by * p.bz;
     if (nb <= 0) return;
@@ -680,10 +692,16 @@ void fill(double *p, long n, double v, hipStream_t s)
     if (n > 0) hipLaunchKernelGGL(K_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
-FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
+FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec, int slab)
 {
     FusedLevel *F = new FusedLevel();
     F->ninst = (int)spec.size();
+    int nzmax = 0;
+    for (const auto &sp : spec) nzmax = std::max(nzmax, sp.n[2]);
+    // slab planes: 0 (or a slab at least as deep as every instance) runs the
+    // face stages over whole instances
+    F->slab = (slab > 0 && slab < nzmax) ? slab : 0;
+    F->nslab = F->slab ? (nzmax + F->slab - 1) / F->slab : 1;
     long offD = 0, offE = 0, offQ = 0;
     for (const auto &sp : spec) {
         FInst I;
@@ -692,12 +710,20 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
             I.dx[d] = sp.dx[d];
             I.ext[d] = sp.n[d] + 2;
         }
+        if (F->slab) I.ext[2] = std::min(F->slab, sp.n[2]) + 2;  // one slab and its ring
+        I.zlo = 0;
+        I.zn = sp.n[2];
         I.periodic = sp.bc == 0;
         I.nD = (long)I.n[0] * I.n[1] * I.n[2];
         I.nE = (long)I.ext[0] * I.ext[1] * I.ext[2];
         I.nE += I.nE & 1;  // even: the species pairs (SP2) stay 16-byte aligned
         if (I.nE >= (1L << 28)) {  // 16-byte lane offsets of the face pairs are 32-bit
-            set_error("fused engine: an instance may hold at most 2^28 extended cells");
+            set_error("fused engine: an instance may hold at most 2^28 extended cells (run it in z-slabs)");
+            delete F;
+            return nullptr;
+        }
+        if (I.nD >= (1L << 29)) {  // 8-byte lane offsets of the cell arrays are 32-bit
+            set_error("fused engine: an instance may hold at most 2^29 cells");
             delete F;
             return nullptr;
         }
@@ -710,6 +736,14 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
         offQ += I.nQ;
         F->hinst.push_back(I);
     }
+    // one instance table per slab: cells [s * slab, (s + 1) * slab) in z,
+    // clipped to each instance (an instance may have no cells in a slab)
+    F->hsinst.assign(F->nslab, F->hinst);
+    for (int sl = 0; sl < F->nslab && F->slab; ++sl)
+        for (FInst &I : F->hsinst[sl]) {
+            I.zlo = sl * F->slab;
+            I.zn = std::max(0, std::min(F->slab, I.n[2] - I.zlo));
+        }
     F->totD = offD;
     F->totE = offE;
     F->totQ = offQ;
@@ -717,6 +751,15 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
     F->tiling[ST_EST] = F->tiling[ST_PRIM] = F->tiling[ST_UPDATE] = KernelTiling{64, 4, 1};
     bool ok = !check_hip(hipMalloc(&F->inst, sizeof(FInst) * F->ninst), "hipMalloc(inst)") &&
               !check_hip(hipMemcpy(F->inst, F->hinst.data(), sizeof(FInst) * F->ninst, hipMemcpyHostToDevice), "h2d");
+    F->sinst.assign(F->nslab, nullptr);
+    if (!F->slab) {
+        F->sinst[0] = F->inst;
+    } else {
+        for (int sl = 0; sl < F->nslab && ok; ++sl)
+            ok = !check_hip(hipMalloc(&F->sinst[sl], sizeof(FInst) * F->ninst), "hipMalloc(slab inst)") &&
+                 !check_hip(hipMemcpy(F->sinst[sl], F->hsinst[sl].data(), sizeof(FInst) * F->ninst,
+                                      hipMemcpyHostToDevice), "h2d");
+    }
     auto A = [&](double *&p, long n) {
         if (ok && check_hip(hipMalloc(&p, sizeof(double) * std::max(n, 1L)), "hipMalloc(fused)")) ok = false;
     };
@@ -755,7 +798,7 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec)
     return F;
 }
 
-FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int bc, const double *dx)
+FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int bc, const double *dx, int slab)
 {
     FusedInstanceSpec sp;
     for (int d = 0; d < 3; ++d) {
@@ -763,7 +806,7 @@ FusedLevel *fused_create(const std::vector<Box> &boxes, const int *n_cell, int b
         sp.dx[d] = dx[d];
     }
     sp.bc = bc;
-    FusedLevel *F = fused_create_set({sp});
+    FusedLevel *F = fused_create_set({sp}, slab);
     if (!F) return nullptr;
     // box arena description for import / export
     std::vector<ArenaBox> ab;
@@ -814,6 +857,9 @@ void fused_destroy(FusedLevel *F)
     if (x.steps) (void)hipFree(x.steps);
     for (auto &pb : F->plans)
         if (pb.blk0) (void)hipFree(pb.blk0);
+    if (F->slab)
+        for (FInst *p : F->sinst)
+            if (p) (void)hipFree(p);
     if (F->inst) (void)hipFree(F->inst);
     if (F->abox) (void)hipFree(F->abox);
     if (F->abstart) (void)hipFree(F->abstart);
@@ -854,6 +900,46 @@ static void for_each_intermediate(FusedArrays &S, long totD, long totE, long tot
     }
     for (int c = 0; c < 6; ++c) f(S.FT[c], NFF * totE);
 }
+
+// device bytes of an engine for these instances: the state, the cell arrays
+// and the face arrays (one slab deep in z when slab > 0)
+size_t fused_bytes_for(const std::vector<FusedInstanceSpec> &spec, int slab)
+{
+    size_t cells = 0, ext = 0, q = 0;
+    for (const auto &sp : spec) {
+        const long nD = (long)sp.n[0] * sp.n[1] * sp.n[2];
+        const int ez = (slab > 0 ? std::min(slab, sp.n[2]) : sp.n[2]) + 2;
+        long nE = (long)(sp.n[0] + 2) * (sp.n[1] + 2) * ez;
+        nE += nE & 1;
+        cells += nD;
+        ext += nE;
+        q += nD + (nD & 1);
+    }
+    const size_t per_cell = NVAR + NU1 + NFQ, per_q = 2 * 2 * NSPEC_PAIRS;
+    const size_t per_ext = 2 + 3 * (2 * NFS + NFF + NFN) + 6 * NFF;
+    return sizeof(double) * (per_cell * cells + per_q * q + per_ext * ext);
+}
+
+// Whole instances when their arrays take at most half the free device
+// memory; else the deepest slab (halving from the deepest instance) whose
+// arrays take at most half of it, but no shallower than 8 planes.
+int fused_auto_slab(const std::vector<FusedInstanceSpec> &spec)
+{
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const size_t budget = freeb / 2;
+    if (fused_bytes_for(spec, 0) <= budget) return 0;
+    int nz = 0;
+    for (const auto &sp : spec) nz = std::max(nz, sp.n[2]);
+    for (int slab = nz / 2; slab >= 8; slab /= 2)
+        if (fused_bytes_for(spec, slab) <= budget) return slab;
+    return 8;
+}
+
+int fused_slab(const FusedLevel *F) { return F->slab; }
 
 size_t fused_intermediate_bytes(const FusedLevel *F)
 {
@@ -945,24 +1031,35 @@ int fused_estimate(FusedLevel *F, hipStream_t s)
     return check_hip(hipGetLastError(), "fused_estimate");
 }
 
-// enqueue one coarse step of every instance
+// the face stages and the update of one z-slab
+static void enqueue_slab(FusedLevel *F, int sl, hipStream_t s)
+{
+    launch_stage(F, ST_FLAT, s, 1, sl);
+    launch_stage(F, ST_TRACE, s, 3, sl);
+    launch_stage(F, ST_FLUX1, s, 3, sl);
+    launch_stage(F, ST_TRANS1, s, 6, sl);
+    launch_stage(F, ST_FINAL, s, 3, sl);
+    launch_stage(F, ST_UPDATE, s, 1, sl);
+}
+
+// enqueue one coarse step of every instance.  K_prim runs over whole
+// instances; then, slab after slab, the face stages and K_update: a slab's
+// update rewrites only its own cells of the state, which no later stage of
+// the step reads (the face stages read the primitives and the cleaned
+// state K_prim stored).
 static void enqueue_step(FusedLevel *F, double stop_time, hipStream_t s)
 {
     F->ctx.tab = eos_table();
     hipLaunchKernelGGL(K_dt, dim3(1), dim3(256), 0, s, F->ctx, F->ninst, stop_time);
     launch_stage(F, ST_PRIM, s);
-    launch_stage(F, ST_FLAT, s);
-    launch_stage(F, ST_TRACE, s, 3);
-    launch_stage(F, ST_FLUX1, s, 3);
-    launch_stage(F, ST_TRANS1, s, 6);
-    launch_stage(F, ST_FINAL, s, 3);
-    launch_stage(F, ST_UPDATE, s);
+    for (int sl = 0; sl < F->nslab; ++sl) enqueue_slab(F, sl, s);
     hipLaunchKernelGGL(K_tick, dim3(1), dim3(256), 0, s, F->ctx, F->ninst);
 }
 
 // Eager steps with a HIP event pair around every stage launch on stream s;
-// stage_ms[st] receives the summed duration of stage st, launches[st] the
-// number of launches (the dt kernels are counted under FST_N).
+// stage_ms[st] receives the summed duration of stage st (over its slabs),
+// launches[st] the number of launches (the dt kernels are counted under
+// FST_N).
 int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, double *stage_ms, int *launches)
 {
     if (!eos_ready()) {
@@ -970,41 +1067,54 @@ int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s
         return -1;
     }
     if (!F->have_est && fused_estimate(F, s)) return -1;
-    static const int order[7] = {ST_PRIM, ST_FLAT, ST_TRACE, ST_FLUX1, ST_TRANS1, ST_FINAL, ST_UPDATE};
+    static const int slab_order[6] = {ST_FLAT, ST_TRACE, ST_FLUX1, ST_TRANS1, ST_FINAL, ST_UPDATE};
     static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
-    const int per = 8;  // 7 stages + the dt kernels
-    std::vector<hipEvent_t> ev(2 * per * (size_t)nsteps);
+    std::vector<int> stage_of;  // stage of each timed launch of one step (NSTAGE: dt control)
+    stage_of.push_back(NSTAGE);
+    stage_of.push_back(ST_PRIM);
+    for (int sl = 0; sl < F->nslab; ++sl)
+        for (int q = 0; q < 6; ++q) stage_of.push_back(slab_order[q]);
+    const size_t per = stage_of.size();
+    std::vector<hipEvent_t> ev(2 * per * (size_t)nsteps, nullptr);
+    int rc = 0;
     for (auto &e : ev)
-        if (check_hip(hipEventCreate(&e), "hipEventCreate")) return -1;
+        if ((rc = check_hip(hipEventCreate(&e), "hipEventCreate"))) break;
     F->ctx.tab = eos_table();
-    for (int n = 0; n < nsteps; ++n) {
+    for (int n = 0; n < nsteps && !rc; ++n) {
         hipEvent_t *E = ev.data() + 2 * per * (size_t)n;
-        (void)hipEventRecord(E[14], s);
+        size_t q = 0;
+        (void)hipEventRecord(E[2 * q], s);
         hipLaunchKernelGGL(K_dt, dim3(1), dim3(256), 0, s, F->ctx, F->ninst, stop_time);
-        (void)hipEventRecord(E[15], s);
-        for (int q = 0; q < 7; ++q) {
-            (void)hipEventRecord(E[2 * q], s);
-            launch_stage(F, order[q], s, ny[order[q]]);
-            (void)hipEventRecord(E[2 * q + 1], s);
-        }
+        (void)hipEventRecord(E[2 * q + 1], s);
+        ++q;
+        (void)hipEventRecord(E[2 * q], s);
+        launch_stage(F, ST_PRIM, s);
+        (void)hipEventRecord(E[2 * q + 1], s);
+        ++q;
+        for (int sl = 0; sl < F->nslab; ++sl)
+            for (int k = 0; k < 6; ++k, ++q) {
+                (void)hipEventRecord(E[2 * q], s);
+                launch_stage(F, slab_order[k], s, ny[slab_order[k]], sl);
+                (void)hipEventRecord(E[2 * q + 1], s);
+            }
         hipLaunchKernelGGL(K_tick, dim3(1), dim3(256), 0, s, F->ctx, F->ninst);
     }
-    int rc = check_hip(hipStreamSynchronize(s), "timed steps");
+    if (!rc) rc = check_hip(hipStreamSynchronize(s), "timed steps");
     for (int st = 0; st <= NSTAGE; ++st) {
         stage_ms[st] = 0.0;
         launches[st] = 0;
     }
     for (int n = 0; n < nsteps && rc == 0; ++n) {
         hipEvent_t *E = ev.data() + 2 * per * (size_t)n;
-        for (int q = 0; q < 8; ++q) {
+        for (size_t q = 0; q < per; ++q) {
             float ms = 0.f;
             if (check_hip(hipEventElapsedTime(&ms, E[2 * q], E[2 * q + 1]), "hipEventElapsedTime")) rc = -1;
-            const int st = q < 7 ? order[q] : NSTAGE;
-            stage_ms[st] += ms;
-            launches[st] += 1;
+            stage_ms[stage_of[q]] += ms;
+            launches[stage_of[q]] += 1;
         }
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
     return rc;
 }
 
@@ -1028,7 +1138,8 @@ int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool
         F->gexec = nullptr;
         F->graph = nullptr;
         // make sure every plan table exists before capture
-        for (int st = 0; st < NSTAGE; ++st) (void)get_plan(F, st);
+        for (int st = 0; st < NSTAGE; ++st)
+            for (int sl = 0; sl < F->nslab; ++sl) (void)get_plan(F, st, sl);
         if (check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) return -1;
         enqueue_step(F, stop_time, s);
         if (check_hip(hipStreamEndCapture(s, &F->graph), "end capture")) return -1;
@@ -1051,12 +1162,7 @@ int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s)
     fill(F->ctx.dt_cur, F->ninst, dt, s);
     fill(F->ctx.dt_est, F->ninst, DBL_MAX, s);
     launch_stage(F, ST_PRIM, s);
-    launch_stage(F, ST_FLAT, s);
-    launch_stage(F, ST_TRACE, s, 3);
-    launch_stage(F, ST_FLUX1, s, 3);
-    launch_stage(F, ST_TRANS1, s, 6);
-    launch_stage(F, ST_FINAL, s, 3);
-    launch_stage(F, ST_UPDATE, s);
+    for (int sl = 0; sl < F->nslab; ++sl) enqueue_slab(F, sl, s);
     F->have_est = true;
     return check_hip(hipGetLastError(), "fused_advance_dt");
 }
@@ -1064,11 +1170,16 @@ int fused_advance_dt(FusedLevel *F, double dt, hipStream_t s)
 // The stages of a step up to the final fluxes with a given dt: recomputes
 // every intermediate array of the step.  K_prim replaces the state by its
 // clean in place (the clean(S_old) a step starts from); no later stage of a
-// step is run, so the state is not advanced.
+// step is run, so the state is not advanced.  The intermediates must cover
+// whole instances (no z-slabs).
 int fused_recompute_dt(FusedLevel *F, double dt, hipStream_t s)
 {
     if (!eos_ready()) {
         set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
+        return -1;
+    }
+    if (F->slab) {
+        set_error("fused_recompute_dt: the engine runs in z-slabs; a step's intermediates are not kept whole");
         return -1;
     }
     fill(F->ctx.dt_cur, F->ninst, dt, s);
@@ -1134,7 +1245,13 @@ int fused_run_stage(FusedLevel *F, int stage, hipStream_t s)
         return -1;
     }
     static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
-    launch_stage(F, stage, s, ny[stage]);
+    if (stage < 0 || stage >= NSTAGE) {
+        set_error("fused_run_stage: no such stage");
+        return -1;
+    }
+    if (whole_instance_stage(stage)) launch_stage(F, stage, s, ny[stage]);
+    else
+        for (int sl = 0; sl < F->nslab; ++sl) launch_stage(F, stage, s, ny[stage], sl);
     return check_hip(hipGetLastError(), "fused_run_stage");
 }
 

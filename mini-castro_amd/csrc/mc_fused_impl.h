@@ -42,7 +42,11 @@ struct FInst {
     long offD, offE;  // cell offsets in the cell / extended index spaces
     long nD, nE;      // cell counts
     long offQ, nQ;    // species pairs of q: offset and cell count, both even
-    int ext[3];       // n + 2
+    int ext[3];       // n + 2 in x and y; in z the planes of one slab + 2
+    // The face stages work on a z-slab of cells [zlo, zlo + zn) (the whole
+    // instance unless the engine runs in slabs): the extended arrays hold the
+    // slab's ring [zlo - 1, zlo + zn] in z, and the launch plans cover it.
+    int zlo, zn;
 };
 
 // Brick tiling of a launch: blocks of bx*by*bz threads, one brick each.
@@ -80,7 +84,7 @@ __device__ __forceinline__ bool plan_cell_at(const Plan &p, int lb, int &m, int 
 {
     m = find_inst(p, lb);
     const FInst &I = p.inst[m];
-    const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.n[2] + p.add;
+    const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.zn + p.add;
     const int nb0 = (e0 + p.bx - 1) / p.bx, nb1 = (e1 + p.by - 1) / p.by;
     const int b = lb - p.blk0[m];
     const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
@@ -89,7 +93,7 @@ __device__ __forceinline__ bool plan_cell_at(const Plan &p, int lb, int &m, int 
     const int li = bi * p.bx + ti, lj = bj * p.by + tj, lk = bk * p.bz + tk;
     i = li - p.shift;
     j = lj - p.shift;
-    k = lk - p.shift;
+    k = lk - p.shift + I.zlo;
     return li < e0 && lj < e1 && lk < e2;
 }
 
@@ -217,11 +221,16 @@ __device__ __forceinline__ idx_t cidx(const FInst &I, int i, int j, int k)
     return (idx_t)bmap(i, I.n[0], I.periodic) +
            (idx_t)I.n[0] * ((idx_t)bmap(j, I.n[1], I.periodic) + (idx_t)I.n[1] * (idx_t)bmap(k, I.n[2], I.periodic));
 }
-// index of an extended cell / face (i,j,k in [-1, n])
+// index of an extended cell / face (i, j in [-1, n], k in the slab's ring
+// [zlo - 1, zlo + zn])
 __device__ __forceinline__ idx_t eidx(const FInst &I, int i, int j, int k)
 {
-    return (idx_t)(i + 1) + (idx_t)I.ext[0] * ((idx_t)(j + 1) + (idx_t)I.ext[1] * (idx_t)(k + 1));
+    return (idx_t)(i + 1) + (idx_t)I.ext[0] * ((idx_t)(j + 1) + (idx_t)I.ext[1] * (idx_t)(k - I.zlo + 1));
 }
+// first cell / face index of direction a the face stages of a slab cover,
+// and the last cell: the instance's bounds in x and y, the slab's in z
+__device__ __forceinline__ int lo_index(const FInst &I, int a) { return a == 2 ? I.zlo : 0; }
+__device__ __forceinline__ int hi_cell(const FInst &I, int a) { return a == 2 ? I.zlo + I.zn - 1 : I.n[a] - 1; }
 // component c of a cell array with C components
 __device__ __forceinline__ gdouble &CA(double *a, const FInst &I, int C, int c, idx_t idx)
 {
@@ -341,12 +350,18 @@ struct FusedLevel {
     // plans: one block table per (tiling, ghost) pair
     struct PlanBuf {
         KernelTiling t;
-        int shift, add;
+        int shift, add, slab;  // slab -1: the whole instance (cell stages)
         int *blk0 = nullptr;
         int nblocks = 0;
     };
     std::vector<PlanBuf> plans;
     KernelTiling tiling[8];  // per stage: est prim flat trace flux1 trans1 final update
+    // z-slabs of the face stages: planes per slab (0: the whole instance),
+    // slab count, and one instance table per slab (zlo, zn set; slab 0 of a
+    // whole-instance engine is `inst` itself)
+    int slab = 0, nslab = 1;
+    std::vector<FInst *> sinst;
+    std::vector<std::vector<FInst>> hsinst;
     bool have_est = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -362,8 +377,9 @@ struct FusedLevel {
 
 enum { ST_EST = FST_EST, ST_PRIM = FST_PRIM, ST_FLAT = FST_FLAT, ST_TRACE = FST_TRACE, ST_FLUX1 = FST_FLUX1,
        ST_TRANS1 = FST_TRANS1, ST_FINAL = FST_FINAL, ST_UPDATE = FST_UPDATE, NSTAGE = FST_N };
-// launch plan of a stage under its current tiling (block table cached)
-Plan get_plan(FusedLevel *F, int stage);
+// launch plan of a stage under its current tiling (block table cached);
+// slab >= 0 selects a z-slab of the face stages, -1 the whole instance
+Plan get_plan(FusedLevel *F, int stage, int slab = 0);
 int plan_blocks(FusedLevel *F, const Plan &p);
 void fill(double *p, long n, double v, hipStream_t s);
 

@@ -166,6 +166,7 @@ struct mc_level {
     // its fine fluxes to, and the register's fine box of each level box (-1)
     mc_flux_register *reg_crse = nullptr, *reg_fine = nullptr;
     std::vector<int> fine_index;
+    int fused_slab_req = -1;  // z-slab planes of the fused engine: -1 chosen by memory, 0 whole level
 };
 
 static double *dalloc(size_t n, bool &ok)
@@ -360,6 +361,24 @@ void mc_level_destroy(mc_level_t *L)
     delete L;
 }
 
+// the level's fused engine, created on first use with the requested z-slabs
+static FusedLevel *level_engine(mc_level_t *L)
+{
+    if (L->fused) return L->fused;
+    int slab = L->fused_slab_req;
+    if (slab < 0) {
+        FusedInstanceSpec sp;
+        for (int d = 0; d < 3; ++d) {
+            sp.n[d] = L->n_cell[d];
+            sp.dx[d] = L->dx[d];
+        }
+        sp.bc = L->bc;
+        slab = fused_auto_slab({sp});
+    }
+    L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx, slab);
+    return L->fused;
+}
+
 static int sync_from_fused(mc_level_t *L)
 {
     if (!L->fused_ahead) return 0;
@@ -371,7 +390,7 @@ static int sync_from_fused(mc_level_t *L)
 // Hand S_new and the step clock to the fused engine.
 static int sync_to_fused(mc_level_t *L)
 {
-    if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+    if (!level_engine(L)) return -1;
     if (fused_import(L->fused, L->S_new, L->stream)) return -1;
     if (fused_set_clock(L->fused, 0, L->dt, L->time, L->steps, L->stream)) return -1;
     if (fused_estimate(L->fused, L->stream)) return -1;
@@ -392,9 +411,31 @@ int mc_level_set_mode(mc_level_t *L, int mode)
     return 0;
 }
 
+int mc_level_set_fused_slab(mc_level_t *L, int planes)
+{
+    if (planes < -1) {
+        set_error("mc_level_set_fused_slab: planes must be -1 (by memory), 0 (whole level) or positive");
+        return -1;
+    }
+    if (L->fused) {  // rebuild the engine, keeping the state and the clock
+        if (sync_from_fused(L)) return -1;
+        fused_destroy(L->fused);
+        L->fused = nullptr;
+    }
+    L->fused_slab_req = planes;
+    if (L->mode == MC_MODE_FUSED && eos_ready()) return sync_to_fused(L);
+    return 0;
+}
+
+int mc_level_fused_slab(mc_level_t *L)
+{
+    if (!level_engine(L)) return -1;
+    return fused_slab(L->fused);
+}
+
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
 {
-    if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+    if (!level_engine(L)) return -1;
     if (stage < 0 || stage >= FST_N || bx <= 0 || by <= 0 || bz <= 0 || (bx * by * bz) % 64 ||
         bx * by * bz > 256) {
         set_error("mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256");
@@ -406,7 +447,7 @@ int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
 
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz)
 {
-    if (!L->fused && !(L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx))) return -1;
+    if (!level_engine(L)) return -1;
     if (stage < 0 || stage >= FST_N) return -1;
     fused_get_tiling(L->fused, stage, bxyz);
     return 0;
@@ -1018,7 +1059,13 @@ extern "C" int mc_advance_batch(const int *n_cell, const double *prob_lo, const 
         E.boxes = boxes;
         std::vector<long> bs(nbox + 1, 0);
         for (int b = 0; b < nbox; ++b) bs[b + 1] = bs[b] + box_npts(boxes[b]);
-        if (!(E.F = fused_create(boxes, n_cell, bc, dx)) ||
+        FusedInstanceSpec sp;
+        for (int d = 0; d < 3; ++d) {
+            sp.n[d] = n_cell[d];
+            sp.dx[d] = dx[d];
+        }
+        sp.bc = bc;
+        if (!(E.F = fused_create(boxes, n_cell, bc, dx, fused_auto_slab({sp}))) ||
             check_hip(hipMalloc(&E.fabs, sizeof(BatchFab) * 2 * nbox), "hipMalloc(batch fabs)") ||
             check_hip(hipMalloc(&E.bstart, sizeof(long) * (nbox + 1)), "hipMalloc(batch starts)") ||
             check_hip(hipMemcpy(E.bstart, bs.data(), sizeof(long) * (nbox + 1), hipMemcpyHostToDevice), "h2d")) {

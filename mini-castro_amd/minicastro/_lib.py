@@ -89,6 +89,8 @@ SIGS = [
     ("mc_level_dt", _D, [_vp]),
     ("mc_level_sync", _I, [_vp]),
     ("mc_level_run", _I, [_vp, _I, _D, _I]),
+    ("mc_level_set_fused_slab", _I, [_vp, _I]),
+    ("mc_level_fused_slab", _I, [_vp]),
     ("mc_level_set_tiling", _I, [_vp, _I, _I, _I, _I]),
     ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
     ("mc_level_run_stage", _I, [_vp, _I]),
@@ -111,6 +113,9 @@ SIGS = [
     ("mc_advance_batch", _I, [_ip, _dp, _dp, _I, _I, C.POINTER(BatchBox), _D, _dp]),
     ("mc_advance_batch_release", None, []),
     ("mc_ensemble_create", _vp, [_I, _ip, _dp, _dp, _ip]),
+    ("mc_ensemble_create_slabs", _vp, [_I, _ip, _dp, _dp, _ip, _I]),
+    ("mc_ensemble_slab", _I, [_vp]),
+    ("mc_fused_bytes", C.c_long, [_I, _ip, _I]),
     ("mc_ensemble_destroy", None, [_vp]),
     ("mc_ensemble_ninst", _I, [_vp]),
     ("mc_ensemble_state_ptr", _dp, [_vp, _I]),

@@ -12,7 +12,9 @@ class Ensemble:
     """instances: list of dicts with n_cell, prob_lo, prob_hi, bc and the
     problem constants p_ambient, dens_ambient, exp_energy, r_init, nsub."""
 
-    def __init__(self, instances):
+    def __init__(self, instances, slab=-1):
+        """slab: z-slab planes of the fused engine's face arrays (-1: chosen
+        by free memory, 0: whole instances)."""
         self.L = lib()
         self.inst = list(instances)
         m = len(self.inst)
@@ -20,10 +22,15 @@ class Ensemble:
         lo = np.ascontiguousarray([x for d in self.inst for x in d["prob_lo"]], dtype=np.float64)
         hi = np.ascontiguousarray([x for d in self.inst for x in d["prob_hi"]], dtype=np.float64)
         bc = np.ascontiguousarray([d["bc"] for d in self.inst], dtype=np.int32)
-        self.h = self.L.mc_ensemble_create(m, nc.ctypes.data_as(_ip), lo.ctypes.data_as(_dp),
-                                           hi.ctypes.data_as(_dp), bc.ctypes.data_as(_ip))
+        self.h = self.L.mc_ensemble_create_slabs(m, nc.ctypes.data_as(_ip), lo.ctypes.data_as(_dp),
+                                                 hi.ctypes.data_as(_dp), bc.ctypes.data_as(_ip), int(slab))
         if not self.h:
             raise RuntimeError("mc_ensemble_create: %s" % self.L.mc_last_error().decode())
+
+    @property
+    def slab(self):
+        """z-slab planes in use (0: whole instances)."""
+        return self.L.mc_ensemble_slab(self.h)
 
     def close(self):
         if getattr(self, "h", None):

@@ -107,6 +107,18 @@ class Level:
         check(self.L.mc_level_run_timed(self.h, int(nsteps), stop_time, ms, nl), "mc_level_run_timed")
         return {self.STAGES[i]: (ms[i], nl[i]) for i in range(9)}
 
+    def set_fused_slab(self, planes):
+        """z-slab planes of the fused engine's face arrays: -1 chosen by free
+        memory (the default), 0 the whole level, > 0 that many cell planes."""
+        check(self.L.mc_level_set_fused_slab(self.h, int(planes)), "mc_level_set_fused_slab")
+
+    @property
+    def fused_slab(self):
+        v = self.L.mc_level_fused_slab(self.h)
+        if v < 0:
+            raise RuntimeError("mc_level_fused_slab: %s" % self.L.mc_last_error().decode())
+        return v
+
     def set_tiling(self, stage, bx, by, bz):
         check(self.L.mc_level_set_tiling(self.h, int(stage), bx, by, bz), "mc_level_set_tiling")
 

@@ -89,13 +89,20 @@ def test_non_cubic_cells_rejected():
 
 
 def test_oversized_instance_rejected_before_allocation():
-    """The fused engine addresses an instance's face pairs with 32-bit
-    16-byte lane offsets and so caps it below 2^28 extended cells: an 812^3
-    instance (814^3 > 2^28) is refused with an error, before any device
-    memory is taken."""
+    """The fused engine addresses an instance's cells with 32-bit 8-byte lane
+    offsets and its face pairs with 32-bit 16-byte offsets: an instance of
+    2^29 cells or more (820^3) is refused, and so is one whose whole-instance
+    face arrays would exceed 2^28 extended cells when z-slabs are turned off
+    (700^3: 702^3 > 2^28), both with an error and before any device memory
+    is taken.  In z-slabs a 700^3 instance is within the limits."""
+    big = dict(prob_lo=(0, 0, 0), prob_hi=(1, 1, 1), bc=1, p_ambient=1e13, dens_ambient=1.0, exp_energy=1e12,
+               r_init=0.1)
+    with pytest.raises(RuntimeError, match="2\\^29"):
+        mc.Ensemble([dict(big, n_cell=(820, 820, 820))])
     with pytest.raises(RuntimeError, match="2\\^28"):
-        mc.Ensemble([dict(n_cell=(812, 812, 812), prob_lo=(0, 0, 0), prob_hi=(1, 1, 1), bc=1,
-                          p_ambient=1e13, dens_ambient=1.0, exp_energy=1e12, r_init=0.1)])
+        mc.Ensemble([dict(big, n_cell=(700, 700, 700))], slab=0)
+    nc = (C.c_int * 3)(700, 700, 700)
+    assert mc.lib().mc_fused_bytes(1, nc, 32) < 0.3 * mc.lib().mc_fused_bytes(1, nc, 0)
 
 
 def test_runs_refuse_without_table_and_recapture_after_reload():

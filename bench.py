@@ -60,15 +60,21 @@ def _stage_bytes(stage, n):
         "flat": 4 * cell + 2 * ring,                               # p, u, v, w in; flatn, div out
         "trace": NQ * cell + ring + 3 * 2 * NS * ring,             # Q, flatn in; minus/plus states out
         "flux1": 3 * (2 * NS + NF) * face + 2 * cell,              # states in, F^d out; c, gamc in
+        # trace + flux1 in one launch: the states it stores are read back by
+        # the same launch, so they count once (written)
+        "tflux": NQ * cell + ring + 3 * 2 * NS * ring + 3 * NF * face,  # Q, flatn in; states, F^d out
         "trans1": 3 * 2 * NS * face + 3 * NF * face + cell + 6 * NF * tface,  # t-states, F^d, gamc in; F^{t|d} out
+        "trans1c": 3 * 2 * NS * face + 3 * NF * face + cell + 6 * NF * tface,  # the same, by cells
         "final": (3 * 2 * NS * face + 6 * NF * tface + ring + 20 * cell + 2 * cell
                   + 3 * NN * fface),                               # states, F^{t|d}, div, U1, c/gamc in; final flux out
+        "finalc": (3 * 2 * NS * face + 6 * NF * tface + ring + 20 * cell + 2 * cell
+                   + 3 * NN * fface),                              # the same, by cells
         "update": 3 * NN * fface + 20 * cell + 20 * cell,          # final fluxes, U in; U out
     }[stage]
     return 8 * doubles
 
 
-STAGES_WITH_BYTES = ("prim", "flat", "trace", "flux1", "trans1", "final", "update")
+STAGES_WITH_BYTES = ("prim", "flat", "trace", "flux1", "tflux", "trans1", "trans1c", "final", "finalc", "update")
 
 
 def measured_traffic(kernel, workload):
@@ -412,6 +418,9 @@ def main():
                     help="also time the CPU baseline step on one thread (about three minutes at 256^3)")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="only time the adjoint workload and print its JSON (for profiling)")
+    ap.add_argument("--stage-variant", type=int, default=0,
+                    help="kernel variant flags of the fused step (0: tflux, trans1c, final; 1: trace + flux1 "
+                         "instead of tflux; 2: trans1 instead of trans1c; 4: finalc instead of final)")
     ap.add_argument("--tiling-json", default=None,
                     help="take the stage tiling from config.tiling of an earlier bench.py JSON line "
                          "instead of running the autotuner")
@@ -445,6 +454,7 @@ def main():
 
     # batched fused path
     L = mc.Level(n, cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_FUSED)
+    L.set_stage_variant(a.stage_variant)
     L.init_data(**cfg)
     if a.tiling_json:
         with open(a.tiling_json) as f:
@@ -455,7 +465,7 @@ def main():
     else:
         tuned = L.autotune(reps=2)   # restores the state it times on
         log("autotuned")
-    tiling = {L.STAGES[st]: list(L.get_tiling(st)) for st in range(1, 8)}
+    tiling = {L.STAGES[st]: list(L.get_tiling(st)) for st in range(1, len(L.STAGES) - 1)}
     L.run(a.warmup, use_graph=True)
     L.sync()
     torch.cuda.synchronize()
@@ -483,20 +493,21 @@ def main():
     # summed measured HBM traffic of the stages of one step (rocprof
     # counters, committed summary) against the zone-update's 320 B.
     w_flop = reference_flop_per_zone_update(cfg["name"])
-    tr = [measured_traffic("mc::K_" + st, cfg["name"]) for st in STAGES_WITH_BYTES]
+    ran = [st for st in STAGES_WITH_BYTES if stages[st][1]]   # the stages this step runs
+    tr = [measured_traffic("mc::K_" + st, cfg["name"]) for st in ran]
     step_traffic = sum(tr) if all(tr) else None
-    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in STAGES_WITH_BYTES]
+    hw = [measured_fp64_ops("mc::K_" + st, cfg["name"]) for st in ran]
     hw_step = sum(hw) if all(hw) else None
     roofline = {
         "bound": "fp64", "unit": "TFLOP/s", "peak": FP64_PEAK / 1e12,
         "achieved": (value * w_flop / 1e12) if w_flop else None,
         "frac": (value * w_flop / FP64_PEAK) if w_flop else None,
         "traffic": step_traffic,
-        "scope": "whole coarse step (7 fused stage launches + dt control); achieved = value x "
-                 "flop_per_zone_update (reference algorithm, counted by the op-count oracle build)",
+        "scope": "whole coarse step (%d fused stage launches + dt control); achieved = value x "
+                 "flop_per_zone_update (reference algorithm, counted by the op-count oracle build)" % len(ran),
         "flop_per_zone_update": w_flop,
-        "traffic_scope": "HBM bytes per step, summed over the 7 stages (rocprof FETCH_SIZE + WRITE_SIZE, "
-                         "calibrated on a plain copy)",
+        "traffic_scope": "HBM bytes per step, summed over the %d stages (rocprof FETCH_SIZE + WRITE_SIZE, "
+                         "calibrated on a plain copy)" % len(ran),
         "traffic_per_zone_update_bytes": (step_traffic / zones) if step_traffic else None,
         "traffic_over_320B_per_zone_update": (step_traffic / zones / B_ALG_PER_ZONE_UPDATE) if step_traffic else None,
         "hbm": {"bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE,
@@ -509,7 +520,7 @@ def main():
 
     # the dominant stage (HIP events on the level's stream) against its own
     # intermediate arrays' bytes -- not the zone-update unit
-    dom = max((k for k in stages if k in STAGES_WITH_BYTES), key=lambda k: stages[k][0])
+    dom = max(ran, key=lambda k: stages[k][0])
     tot_ms, nl = stages[dom]
     avg_s = tot_ms / nl / 1e3
     traffic_alg = _stage_bytes(dom, n)
@@ -565,6 +576,8 @@ def main():
                       "graph_replay_ms_per_step": graph_ms, "steps_advanced": fused_steps_done,
                       "fom_zones_per_usec": value / 1e6,
                       "tiling": tiling, "tiling_source": "autotuner" if tuned else a.tiling_json,
+                      "stage_variant": a.stage_variant,
+                      "stages": [st for st in STAGES_WITH_BYTES if stages[st][1]],
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,
                       "batch_entry": batch,

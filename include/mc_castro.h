@@ -248,8 +248,9 @@ double mc_level_coarse_step(mc_level_t *L, double stop_time);
    the step may be replayed from a captured hipGraph (use_graph != 0). */
 int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph);
 /* Fused mode: nsteps eager steps with HIP events around every stage launch on
-   the level's stream; stage_ms[0..8] / launches[0..8] receive per-stage totals
-   (index 8: the time-step control kernels). */
+   the level's stream; stage_ms[0..11] / launches[0..11] receive per-stage
+   totals (stages as in mc_level_set_tiling; index 11: the time-step control
+   kernels).  A stage the step does not run reports 0 launches. */
 int mc_level_run_timed(mc_level_t *L, int nsteps, double stop_time, double *stage_ms, int *launches);
 int mc_level_steps(const mc_level_t *L);
 /* Castro::post_timestep's blast-radius diagnostic (Source/Castro.cpp:371-424):
@@ -271,8 +272,13 @@ int mc_level_sync(mc_level_t *L);
    mc_level_fused_slab returns the planes in use (0: whole level). */
 int mc_level_set_fused_slab(mc_level_t *L, int planes);
 int mc_level_fused_slab(mc_level_t *L);
-/* Fused-mode tiling: stage 0..7 = est, prim, flat, trace, flux1, trans1,
-   final, update; a brick of bx*by*bz threads (multiple of 64, <= 256). */
+/* Fused-mode tiling: stage 0..10 = est, prim, flat, trace, flux1, trans1,
+   final, update, tflux, trans1c, finalc; a brick of bx*by*bz threads
+   (multiple of 64, <= 256).  tflux (trace and flux1 in one launch), trans1c
+   and finalc (trans1 and final by cells), whose bricks overlap by one cell
+   along their direction, take (W, L, X): bricks (W, L, 1) along y,
+   (W, 1, L) along z, (X, W*L/X, 1) along x; L, X >= 2, W*L threads
+   (multiple of 64, <= 256), X dividing W*L. */
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
 /* Launch one fused stage on the current device state (for timing).  The
@@ -281,10 +287,22 @@ int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
 int mc_level_run_stage(mc_level_t *L, int stage);
 /* Fused mode: time every candidate brick of every stage of the step (reps
    launches each) on the data of one coarse step and keep the fastest; the
-   state and the clock are restored.  bricks (8 x 3 ints) and best_ms (8)
-   receive the chosen tiling and its time per launch (stage 0, est, is not
-   tuned); either may be NULL. */
+   state and the clock are restored.  bricks (11 x 3 ints) and best_ms (11)
+   receive every stage's tiling and the tuned stages' time per launch (0 for
+   a stage not tuned: est, and the stages the step does not run); either may
+   be NULL. */
 int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms);
+/* Fused mode: kernel variants of the step, bit flags (0, the default, runs
+   the stages tflux, trans1c and final): MC_VARIANT_SPLIT_TRACE runs the
+   traces and the first-stage fluxes as two launches (trace, flux1) instead
+   of one (tflux); MC_VARIANT_FACE_TRANS1 the transverse-flux stage face by
+   face (trans1) instead of cell by cell (trans1c); MC_VARIANT_CELL_FINAL
+   the final-flux stage cell by cell (finalc) instead of face by face
+   (final).  Results are the same bit for bit. */
+#define MC_VARIANT_SPLIT_TRACE 1
+#define MC_VARIANT_FACE_TRANS1 2
+#define MC_VARIANT_CELL_FINAL 4
+int mc_level_set_stage_variant(mc_level_t *L, int flags);
 
 /* ---- AMR coupling of a coarse and a fine level ----
    The flux register of a fine level (AMReX FluxRegister as the reference

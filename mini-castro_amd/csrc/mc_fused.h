@@ -16,7 +16,14 @@ struct FusedInstanceSpec {
     int bc;  // 0 periodic, 1 outflow
 };
 
-enum { FST_EST = 0, FST_PRIM, FST_FLAT, FST_TRACE, FST_FLUX1, FST_TRANS1, FST_FINAL, FST_UPDATE, FST_N };
+// Stages of a step.  FST_TFLUX is FST_TRACE and FST_FLUX1 in one launch,
+// FST_TRANS1C and FST_FINALC are FST_TRANS1 and FST_FINAL computed by
+// cells; which of them a step runs is its kernel variant
+// (fused_set_variant).
+enum {
+    FST_EST = 0, FST_PRIM, FST_FLAT, FST_TRACE, FST_FLUX1, FST_TRANS1, FST_FINAL, FST_UPDATE, FST_TFLUX, FST_TRANS1C,
+    FST_FINALC, FST_N
+};
 
 // slab > 0: the face stages run in z-slabs of that many cell planes, so the
 // face arrays hold one slab (and its ring) instead of whole instances; 0: whole
@@ -43,10 +50,21 @@ int fused_recompute_dt(FusedLevel *F, double dt, hipStream_t s);
 int fused_set_clock(FusedLevel *F, int m, double dt, double time, int steps, hipStream_t s);
 int fused_read_clock(FusedLevel *F, int m, double *dt, double *time, int *steps, hipStream_t s);
 int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s);
-void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz);
+// a brick of 64..256 threads (a multiple of 64); for FST_TFLUX,
+// FST_TRANS1C and FST_FINALC the triple (W, L, X) with L, X >= 2, W*L of
+// 64..256 threads and X dividing W*L
+bool fused_tiling_ok(int stage, int bx, int by, int bz);
+void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz);  // ignores an invalid tiling
 void fused_get_tiling(FusedLevel *F, int stage, int *bxyz);
 int fused_run_stage(FusedLevel *F, int stage, hipStream_t s);
 int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *best_ms);
+// Kernel variants of a step (bit flags; 0, the default, runs K_tflux,
+// K_trans1c and K_final): FV_SPLIT_TRACE runs K_trace + K_flux1 instead of
+// K_tflux, FV_FACE_TRANS1 K_trans1 instead of K_trans1c, FV_CELL_FINAL
+// K_finalc instead of K_final.  Results are the same bit for bit.
+enum { FV_SPLIT_TRACE = 1, FV_FACE_TRANS1 = 2, FV_CELL_FINAL = 4, FV_ALL = 7 };
+void fused_set_variant(FusedLevel *F, int flags);
+int fused_variant(const FusedLevel *F);
 
 // The arrays a step writes besides the state (primitives, flattening,
 // traces, fluxes).  A caller may give the engine another set to write into,

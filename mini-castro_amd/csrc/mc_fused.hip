@@ -171,116 +171,168 @@ __global__ __launch_bounds__(256) void K_flat(Plan p, Ctx x)
     }
 }
 
-// PPM reconstruction + characteristic tracing (Source/ppm.F90:309-743)
-__global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
+// What the species part of a cell's trace takes from its hydro part: the
+// stencil, the wave speed and dt/dx, which states the cell stores, and the
+// species pair bases (taken before any store, see SpeciesPairs).
+struct TraceCtx {
+    idx_t cs[5], ep, em;
+    double flatn, un, dtdx;
+    bool do_p, do_m;
+    SpeciesPairs XP, XM, XQ;
+};
+
+// hydro part of a traced state, velocities in absolute order
+struct StateH {
+    double rho, u, v, w, p, rhoe;
+};
+
+// PPM reconstruction + characteristic tracing (Source/ppm.F90:309-743) of
+// cell (i,j,k) of instance m in direction d, hydro part: the plus state of
+// face pos (stored when own_p and pos >= 0) and the minus state of face
+// pos + 1, both also returned as the Riemann solver of direction d takes
+// them (fp, fm: the values load_fs_h reads back from the stored states)
+__device__ __forceinline__ void trace_hydro(const Ctx &x, int m, const FInst &I, int d, int i, int j, int k,
+                                            bool own_p, TraceCtx &t, FaceH &fp, FaceH &fm)
 {
-    int m, i, j, k, sub_;
-    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
-    const int d = sub_;
-    const FInst &I = p.inst[m];
     const int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const int pos = d == 0 ? i : (d == 1 ? j : k);
-    const bool do_p = pos >= 0;
+    t.do_p = own_p && pos >= 0;
     // the minus state of face pos + 1; in z it must lie in the slab's faces
-    const bool do_m = pos <= I.n[d] - 1 && (d != 2 || k + 1 <= I.zlo + I.zn);
+    t.do_m = pos <= I.n[d] - 1 && (d != 2 || k + 1 <= I.zlo + I.zn);
     const int cun = FQ_U + d, cut = d == 0 ? FQ_V : (d == 1 ? FQ_W : FQ_U), cutt = d == 0 ? FQ_W : (d == 1 ? FQ_U : FQ_V);
     const idx_t c0 = cidx(I, i, j, k);
     const double cc = CA(x.Q, I, NFQ, FQ_C, c0);
     const double un = CA(x.Q, I, NFQ, cun, c0);
     const double flatn = EA(x.FL, I, 1, 0, eidx(I, i, j, k));
     const double dtdx = x.dt_cur[m] / I.dx[d];
-    idx_t cs[5];
+    t.un = un;
+    t.flatn = flatn;
+    t.dtdx = dtdx;
 #pragma unroll
-    for (int s = -2; s <= 2; ++s) cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
+    for (int s = -2; s <= 2; ++s) t.cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
 
     double Ip[3][4], Im[3][4], Ip_ut = 0, Im_ut = 0, Ip_utt = 0, Im_utt = 0;
     const int vars[6] = {FQ_RHO, cun, FQ_P, FQ_RE, cut, cutt};
     for (int vv = 0; vv < 6; ++vv) {
-        double s[5], sm, sp;
+        double s[5], sm_, sp_;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, vars[vv], cs[q]);
-        ppm_reconstruct(s, flatn, sm, sp);
-        const double s6 = ppm_s6(sm, sp, s[2]);
+        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, vars[vv], t.cs[q]);
+        ppm_reconstruct(s, flatn, sm_, sp_);
+        const double s6 = ppm_s6(sm_, sp_, s[2]);
         if (vv < 4) {
-            ppm_int_wave(sm, sp, s6, un - cc, dtdx, Ip[0][vv], Im[0][vv]);
-            ppm_int_wave(sm, sp, s6, un, dtdx, Ip[1][vv], Im[1][vv]);
-            ppm_int_wave(sm, sp, s6, un + cc, dtdx, Ip[2][vv], Im[2][vv]);
+            ppm_int_wave(sm_, sp_, s6, un - cc, dtdx, Ip[0][vv], Im[0][vv]);
+            ppm_int_wave(sm_, sp_, s6, un, dtdx, Ip[1][vv], Im[1][vv]);
+            ppm_int_wave(sm_, sp_, s6, un + cc, dtdx, Ip[2][vv], Im[2][vv]);
         } else if (vv == 4) {
-            ppm_int_wave(sm, sp, s6, un, dtdx, Ip_ut, Im_ut);
+            ppm_int_wave(sm_, sp_, s6, un, dtdx, Ip_ut, Im_ut);
         } else {
-            ppm_int_wave(sm, sp, s6, un, dtdx, Ip_utt, Im_utt);
+            ppm_int_wave(sm_, sp_, s6, un, dtdx, Ip_utt, Im_utt);
         }
     }
     double gam_m, gam_p;
     {
-        double s[5], sm, sp, a, b;
+        double s[5], sm_, sp_, a, b;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, FQ_G, cs[q]);
-        ppm_reconstruct(s, flatn, sm, sp);
-        const double s6 = ppm_s6(sm, sp, s[2]);
-        ppm_int_wave(sm, sp, s6, un - cc, dtdx, a, gam_m);
-        ppm_int_wave(sm, sp, s6, un + cc, dtdx, gam_p, b);
+        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, FQ_G, t.cs[q]);
+        ppm_reconstruct(s, flatn, sm_, sp_);
+        const double s6 = ppm_s6(sm_, sp_, s[2]);
+        ppm_int_wave(sm_, sp_, s6, un - cc, dtdx, a, gam_m);
+        ppm_int_wave(sm_, sp_, s6, un + cc, dtdx, gam_p, b);
         (void)a;
         (void)b;
     }
-    const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + di, j + dj, k + dk);
+    t.ep = eidx(I, i, j, k);
+    t.em = eidx(I, i + di, j + dj, k + dk);
     double *SP = x.SP[d], *SM = x.SM[d];
-    const SpeciesPairs XP = species_pairs(SP, I, NFS, FS_X), XM = species_pairs(SM, I, NFS, FS_X);
-    const SpeciesPairs XQ = cell_species(x.QX, I);
+    t.XP = species_pairs(SP, I, NFS, FS_X);
+    t.XM = species_pairs(SM, I, NFS, FS_X);
+    t.XQ = cell_species(x.QX, I);
     // velocities back in absolute order: the normal one is FS_U + d, the
     // first transverse FS_V, FS_W, FS_U and the second FS_W, FS_U, FS_V for
     // d = 0, 1, 2; stored as the pairs (rho, u), (v, w), (p, rhoe)
-    auto store_state = [&](double *S, idx_t ee, double r, double un_, double ut_, double utt_, double e_, double p_) {
-        const double vx = d == 0 ? un_ : (d == 1 ? utt_ : ut_);
-        const double vy = d == 0 ? ut_ : (d == 1 ? un_ : utt_);
-        const double vz = d == 0 ? utt_ : (d == 1 ? ut_ : un_);
-        EF2(S, I, NFS, FS_RHO, ee) = dbl2{r, vx};
-        EF2(S, I, NFS, FS_V, ee) = dbl2{vy, vz};
-        EF2(S, I, NFS, FS_P, ee) = dbl2{p_, e_};
+    // the solver's transverse velocities are (v, w), (u, w), (u, v) for
+    // d = 0, 1, 2: the traced (ut, utt) except at d = 1, where they swap
+    auto to_abs = [&](StateH &o, FaceH &f, double r, double un_, double ut_, double utt_, double e_, double p_) {
+        o.rho = r;
+        o.u = d == 0 ? un_ : (d == 1 ? utt_ : ut_);
+        o.v = d == 0 ? ut_ : (d == 1 ? un_ : utt_);
+        o.w = d == 0 ? utt_ : (d == 1 ? ut_ : un_);
+        o.p = p_;
+        o.rhoe = e_;
+        f.rho = r;
+        f.un = un_;
+        f.ut = d == 1 ? utt_ : ut_;
+        f.utt = d == 1 ? ut_ : utt_;
+        f.p = p_;
+        f.rhoe = e_;
     };
-    if (do_p) {
+    auto store_state = [&](double *S, idx_t ee, const StateH &o) {
+        EF2(S, I, NFS, FS_RHO, ee) = dbl2{o.rho, o.u};
+        EF2(S, I, NFS, FS_V, ee) = dbl2{o.v, o.w};
+        EF2(S, I, NFS, FS_P, ee) = dbl2{o.p, o.rhoe};
+    };
+    {
         double r, u_, e, pp;
         ppm_project<true>(Im, gam_m, un, cc, r, u_, e, pp);
-        store_state(SP, ep, r, u_, Im_ut, Im_utt, e, pp);
+        StateH sp;
+        to_abs(sp, fp, r, u_, Im_ut, Im_utt, e, pp);
+        if (t.do_p) store_state(SP, t.ep, sp);
     }
-    if (do_m) {
+    {
         double r, u_, e, pp;
         ppm_project<false>(Ip, gam_p, un, cc, r, u_, e, pp);
-        store_state(SM, em, r, u_, Ip_ut, Ip_utt, e, pp);
-    }
-    // species last: their stores come after every read of the instance table
-    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
-        double ipv[2] = {0.0, 0.0}, imv[2] = {0.0, 0.0};
-        dbl2 s2[5];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) s2[q] = SP2(XQ, pr, cs[q]);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int ns = 2 * pr + h;
-            if (ns >= NSPEC) break;
-            double s[5], sm, sp;
-#pragma unroll
-            for (int q = 0; q < 5; ++q) s[q] = h == 0 ? s2[q].x : s2[q].y;
-            ppm_reconstruct(s, flatn, sm, sp);
-            const double s6 = ppm_s6(sm, sp, s[2]);
-            ppm_int_wave(sm, sp, s6, un, dtdx, ipv[h], imv[h]);
-        }
-        if (do_p) SP2(XP, pr, ep) = dbl2{imv[0], imv[1]};
-        if (do_m) SP2(XM, pr, em) = dbl2{ipv[0], ipv[1]};
+        StateH sm;
+        to_abs(sm, fm, r, u_, Ip_ut, Ip_utt, e, pp);
+        if (t.do_m) store_state(SM, t.em, sm);
     }
 }
 
-// Source/riemann.F90:10-331 on the PPM states: F^d on d-faces
-// (d-face index 0..n_d, transverse cells -1..n).  Species are streamed in
-// pairs.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K_flux1(Plan p, Ctx x)
+// Species pair pr of the trace (after trace_hydro): the plus-state pair of
+// face pos in xp, the minus-state pair of face pos + 1 in xm, both stored
+// where the cell stores its states.  The last pair of an odd species count
+// carries 0 in its second slot.
+__device__ __forceinline__ void trace_species(const TraceCtx &t, int pr, dbl2 &xp, dbl2 &xm)
+{
+    double ipv[2] = {0.0, 0.0}, imv[2] = {0.0, 0.0};
+    dbl2 s2[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) s2[q] = SP2(t.XQ, pr, t.cs[q]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ns = 2 * pr + h;
+        if (ns >= NSPEC) break;
+        double s[5], sm, sp;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s[q] = h == 0 ? s2[q].x : s2[q].y;
+        ppm_reconstruct(s, t.flatn, sm, sp);
+        const double s6 = ppm_s6(sm, sp, s[2]);
+        ppm_int_wave(sm, sp, s6, t.un, t.dtdx, ipv[h], imv[h]);
+    }
+    xp = dbl2{imv[0], imv[1]};
+    xm = dbl2{ipv[0], ipv[1]};
+    if (t.do_p) SP2(t.XP, pr, t.ep) = xp;
+    if (t.do_m) SP2(t.XM, pr, t.em) = xm;
+}
+
+__global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
 {
     int m, i, j, k, sub_;
     if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
-    const int d = sub_;
-    const FInst &I = p.inst[m];
-    const int pos = d == 0 ? i : (d == 1 ? j : k);
-    if (pos < lo_index(I, d)) return;
+    TraceCtx t;
+    FaceH fp, fm;
+    trace_hydro(x, m, p.inst[m], sub_, i, j, k, true, t, fp, fm);
+    // species last: their stores come after every read of the instance table
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        dbl2 xp, xm;
+        trace_species(t, pr, xp, xm);
+    }
+}
+
+// Source/riemann.F90:10-331 on the PPM states: F^d on d-face (i,j,k)
+// (d-face index 0..n_d, transverse cells -1..n).  Species are streamed in
+// pairs.
+__device__ __forceinline__ void flux1_face(const Ctx &x, const FInst &I, int d, int i, int j, int k)
+{
     const idx_t e = eidx(I, i, j, k);
     FaceH L, R;
     load_fs_h(x.SM[d], I, e, d, L);
@@ -301,6 +353,101 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K
         const double gx0 = o.fpl * xl.x + o.fmr * xr.x;
         const double gx1 = o.fpl * xl.y + o.fmr * xr.y;
         SP2(XF, pr, e) = dbl2{o.f_rho * gx0, o.f_rho * gx1};
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K_flux1(Plan p, Ctx x)
+{
+    int m, i, j, k, sub_;
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
+    const int d = sub_;
+    const FInst &I = p.inst[m];
+    const int pos = d == 0 ? i : (d == 1 ? j : k);
+    if (pos < lo_index(I, d)) return;
+    flux1_face(x, I, d, i, j, k);
+}
+
+// the thread of the layer below along a brick's overlap direction (stride st
+// in threads); only read for threads of layer l >= 1
+__device__ __forceinline__ int below_of(int tid, int st) { return tid - st; }
+
+// K_trace and K_flux1 in one launch.  The bricks of direction d overlap
+// along d by one cell layer (plan_cell_ovl).  Every thread traces its cell
+// and stores both states as K_trace does; the minus state (for the cell's
+// high d-face) also goes to LDS, and after the block's barrier the thread
+// of each layer l >= 1 solves the Riemann problem on its cell's low d-face
+// with the minus state the layer below left in LDS and its own plus state,
+// kept in registers: the first-stage flux reads no traced state from
+// memory.  The species follow the same way, all pairs at once.
+// A layer-0 cell belongs to the brick below as its top layer: it stores its
+// minus state (the same values that brick stores) but not its plus state,
+// and solves no Riemann problem.  The sub-launches are the directions,
+// interleaved block by block like the other face stages.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K_tflux(Plan3 p3, Ctx x)
+{
+    // the minus states' hydro part, then (after the barrier that ends its
+    // reads) their species in the same memory
+    __shared__ union {
+        FaceH h[256];
+        dbl2 s[NSPEC_PAIRS][256];
+    } lds;
+    FaceH(&lds_h)[256] = lds.h;
+    dbl2(&lds_s)[NSPEC_PAIRS][256] = lds.s;
+    int lb = logical_block(), d;
+    if (gridDim.y > 1) {
+        d = blockIdx.y;
+    } else {
+        d = lb % 3;
+        lb /= 3;
+    }
+    const Plan p = d == 0 ? p3.p[0] : (d == 1 ? p3.p[1] : p3.p[2]);
+    if (lb >= (d == 0 ? p3.nb[0] : (d == 1 ? p3.nb[1] : p3.nb[2]))) return;  // the whole block
+    int m, i, j, k, l;
+    const bool ok = plan_cell_ovl(p, d, lb, m, i, j, k, l);
+    const FInst &I = p.inst[m];
+    const int tid = threadIdx.x;
+    const int below = tid - (d == 0 ? 1 : (d == 1 ? p.bx : p.bx * p.by));  // the thread of layer l - 1
+    const int pos = d == 0 ? i : (d == 1 ? j : k);
+    const bool fx = ok && l > 0 && pos >= lo_index(I, d);
+    TraceCtx t;
+    FaceH fp, fm;
+    if (ok) {
+        trace_hydro(x, m, I, d, i, j, k, l > 0, t, fp, fm);
+        lds_h[tid] = fm;
+    }
+    const SpeciesPairs XF = species_pairs(x.FD[d], I, NFF, FF_X);
+    __syncthreads();
+    RiemannH o;
+    bool useL = false, useR = false;
+    if (fx) {
+        const FaceH L = lds_h[below];
+        face_riemann_h(x, I, d, i, j, k, L, fp, o);
+        store_tflux_h(x.FD[d], I, t.ep, d, o);
+        // a side whose weight is zero contributes +-0 to fpl*xl + fmr*xr
+        useL = o.fpl != 0.0;
+        useR = o.fmr != 0.0;
+    }
+    __syncthreads();  // the hydro minus states are read: LDS takes the species
+    // every pair's species trace first (the minus states to LDS, the plus
+    // states kept), so that their loads are in flight together; one barrier
+    dbl2 xpv[NSPEC_PAIRS];
+#pragma unroll
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        dbl2 xp = dbl2{0.0, 0.0}, xm = dbl2{0.0, 0.0};
+        if (ok) trace_species(t, pr, xp, xm);
+        lds_s[pr][tid] = xm;
+        xpv[pr] = xp;
+    }
+    __syncthreads();
+    if (fx) {
+#pragma unroll
+        for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+            const dbl2 xl = useL ? lds_s[pr][below] : dbl2{0.0, 0.0};
+            const dbl2 xr = useR ? xpv[pr] : dbl2{0.0, 0.0};
+            const double gx0 = o.fpl * xl.x + o.fmr * xr.x;
+            const double gx1 = o.fpl * xl.y + o.fmr * xr.y;
+            SP2(XF, pr, t.ep) = dbl2{o.f_rho * gx0, o.f_rho * gx1};
+        }
     }
 }
 
@@ -368,6 +515,196 @@ __global__ __launch_bounds__(256) void K_trans1(Plan p, Ctx x)
     }
 }
 
+// K_trans1 by cells: the thread of cell c (bricks elongated along the
+// state direction t, overlapping by one cell, as K_tflux) applies trans1 to
+// both t-states of its cell, the plus state of t-face c and the minus state
+// of t-face c + 1, with the d-fluxes on its cell's two d-faces, which it
+// reads once for both (K_trans1 reads them once per face side).  The minus
+// side goes through LDS to the thread above, which solves the Riemann
+// problem on its t-face as K_trans1 does.  Species: a face's weights decide
+// which sides it reads (the minus side's thread learns it through LDS), and
+// the corrected minus-side species follow pair by pair through LDS.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K_trans1c(Plan3 p3, Ctx x)
+{
+    // the minus sides' hydro part, then (after the barrier that ends its
+    // reads) their species in the same memory
+    __shared__ union {
+        FaceH h[256];
+        dbl2 s[NSPEC_PAIRS][256];
+    } lds;
+    __shared__ unsigned char lds_f[256];
+    FaceH(&lds_h)[256] = lds.h;
+    dbl2(&lds_s)[NSPEC_PAIRS][256] = lds.s;
+    int lb = logical_block(), combo;
+    if (gridDim.y > 1) {
+        combo = blockIdx.y;
+    } else {
+        combo = lb % 6;
+        lb /= 6;
+    }
+    const int t = c_pair_t[combo], d = c_pair_d[combo];
+    const Plan p = t == 0 ? p3.p[0] : (t == 1 ? p3.p[1] : p3.p[2]);
+    if (lb >= (t == 0 ? p3.nb[0] : (t == 1 ? p3.nb[1] : p3.nb[2]))) return;  // the whole block
+    int m, i, j, k, l;
+    const bool ok = plan_cell_ovl(p, t, lb, m, i, j, k, l);
+    const FInst &I = p.inst[m];
+    const int tid = threadIdx.x;
+    const int bt = t == 0 ? p.bx : (t == 1 ? p.by : p.bz);
+    const int st = t == 0 ? 1 : (t == 1 ? p.bx : p.bx * p.by);
+    const int ct = t == 0 ? i : (t == 1 ? j : k), cd = d == 0 ? i : (d == 1 ? j : k);
+    // t-faces lo_index..n_t (the slab's top in z), d-cells lo_index..hi_cell
+    const int tmax = t == 2 ? I.zlo + I.zn : I.n[t];
+    const bool act = ok && cd >= lo_index(I, d) && cd <= hi_cell(I, d);
+    const bool has_p = act && l > 0 && ct >= lo_index(I, t);  // t-face c (the plus side)
+    const bool has_m = act && l + 1 < bt && ct + 1 <= tmax;    // t-face c + 1 (the minus side)
+    const int et0 = t == 0, et1 = t == 1, et2 = t == 2, ed0 = d == 0, ed1 = d == 1, ed2 = d == 2;
+    const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + et0, j + et1, k + et2);
+    const idx_t fl_i = ep, fr_i = eidx(I, i + ed0, j + ed1, k + ed2);  // the cell's d-faces
+    const double cdtdx = x.dt_cur[m] / I.dx[d];
+    const double cdt = cdtdx / 3.0;
+    double *F = x.FT[combo], *FDd = x.FD[d];
+    const SpeciesPairs XL = species_pairs(x.SM[t], I, NFS, FS_X), XR = species_pairs(x.SP[t], I, NFS, FS_X),
+                       XD = species_pairs(FDd, I, NFF, FF_X), XF = species_pairs(F, I, NFF, FF_X);
+    FaceH side_p;
+    double rho_p = 0.0, rrnew_p = 0.0, rho_m = 0.0, rrnew_m = 0.0;
+    if (has_p || has_m) {
+        double fr[UEDEN + 1], fl[UEDEN + 1], pgp, ugp, pgm, ugm;
+        load_tflux_h(FDd, I, fr_i, fr, pgp, ugp);
+        load_tflux_h(FDd, I, fl_i, fl, pgm, ugm);
+        const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, i, j, k));
+        if (has_m) {
+            QInH lq;
+            load_qin_h(x.SM[t], I, em, lq);
+            const QStateH o = trans1_hydro(lq, fr, fl, pgp, pgm, ugp, ugm, gamc, cdt);
+            rho_m = lq.rho;
+            rrnew_m = lq.rho - cdt * (fr[URHO] - fl[URHO]);
+            lds_h[tid] = to_face(o, t);
+        }
+        if (has_p) {
+            QInH lq;
+            load_qin_h(x.SP[t], I, ep, lq);
+            const QStateH o = trans1_hydro(lq, fr, fl, pgp, pgm, ugp, ugm, gamc, cdt);
+            rho_p = lq.rho;
+            rrnew_p = lq.rho - cdt * (fr[URHO] - fl[URHO]);
+            side_p = to_face(o, t);
+        }
+    }
+    __syncthreads();
+    RiemannH o;
+    bool useL = false, useR = false;
+    if (has_p) {
+        face_riemann_h(x, I, t, i, j, k, lds_h[below_of(tid, st)], side_p, o);
+        store_tflux_h(F, I, ep, t, o);
+        useL = o.fpl != 0.0;
+        useR = o.fmr != 0.0;
+    }
+    lds_f[tid] = useL;
+    __syncthreads();
+    const bool need_m = has_m && lds_f[tid + st];  // the face above reads this minus side
+    // every pair's corrected species first (the minus side to LDS, the plus
+    // side kept), so that their loads are in flight together; one barrier
+    dbl2 xpv[NSPEC_PAIRS];
+#pragma unroll
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const bool two = 2 * pr + 1 < NSPEC;  // the last pair holds one species
+        double xm[2] = {0.0, 0.0}, xp[2] = {0.0, 0.0};
+        if (need_m || useR) {
+            const dbl2 fr = SP2(XD, pr, fr_i), fl = SP2(XD, pr, fl_i);
+            if (need_m) {
+                const dbl2 s = SP2(XL, pr, em);
+                xm[0] = trans1_species(rho_m, s.x, fr.x, fl.x, cdt, rrnew_m);
+                if (two) xm[1] = trans1_species(rho_m, s.y, fr.y, fl.y, cdt, rrnew_m);
+            }
+            if (useR) {
+                const dbl2 s = SP2(XR, pr, ep);
+                xp[0] = trans1_species(rho_p, s.x, fr.x, fl.x, cdt, rrnew_p);
+                if (two) xp[1] = trans1_species(rho_p, s.y, fr.y, fl.y, cdt, rrnew_p);
+            }
+        }
+        lds_s[pr][tid] = dbl2{xm[0], xm[1]};
+        xpv[pr] = dbl2{xp[0], xp[1]};
+    }
+    __syncthreads();
+    if (has_p) {
+#pragma unroll
+        for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+            const bool two = 2 * pr + 1 < NSPEC;
+            const dbl2 x0 = useL ? lds_s[pr][below_of(tid, st)] : dbl2{0.0, 0.0};
+            SP2(XF, pr, ep) = dbl2{o.f_rho * (o.fpl * x0.x + o.fmr * xpv[pr].x),
+                                   two ? o.f_rho * (o.fpl * x0.y + o.fmr * xpv[pr].y) : 0.0};
+        }
+    }
+}
+
+// The artificial viscosity of d-face (i,j,k) and the hydro part of its final
+// flux (apply_av, Source/Castro_hydro_nd.F90:174-245, with the reference's
+// corner-node pattern): stores the conserved fluxes and the Godunov p of
+// face e and returns what the species fluxes need.
+struct FinalAV {
+    double div1, dxd, frho;
+    idx_t cr, cl;
+};
+__device__ __forceinline__ FinalAV final_hydro(const Ctx &x, const FInst &I, int d, int i, int j, int k, idx_t e,
+                                               const RiemannH &o)
+{
+    FinalAV v;
+    double div1;
+#define DVN(a, b, cc) EA(x.DV, I, 1, 0, eidx(I, a, b, cc))
+    if (d == 0) div1 = FOURTH * (DVN(i, j, k) + DVN(i, j + 1, k) + DVN(i, j, k + 1) + DVN(i, j + 1, k));
+    else if (d == 1) div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j, k + 1) + DVN(i + 1, j, k));
+    else div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j + 1, k) + DVN(i + 1, j + 1, k));
+#undef DVN
+    v.div1 = 0.1 * fmin(0.0, div1);
+    v.cr = cidx(I, i, j, k);
+    v.cl = cidx(I, i - (d == 0), j - (d == 1), k - (d == 2));
+    v.dxd = I.dx[d];
+#define AVN(n, fval) ((fval) + v.dxd * (v.div1 * (CA(x.U1, I, NU1, n, v.cr) - CA(x.U1, I, NU1, n, v.cl))))
+    const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
+    double *FN = x.FN[d];
+    v.frho = AVN(URHO, o.f_rho);
+    // momenta back in absolute order (mn = UMX + d, mt, mtt as above)
+    const double vmn = AVN(mn, o.f_mn), vmt = AVN(mt, o.f_mt), vmtt = AVN(mtt, o.f_mtt);
+    const double mx = d == 0 ? vmn : vmt;
+    const double my = d == 0 ? vmt : (d == 1 ? vmn : vmtt);
+    const double mz = d == 2 ? vmn : vmtt;
+    EF2(FN, I, NFN, URHO, e) = dbl2{v.frho, mx};
+    EF2(FN, I, NFN, UMY, e) = dbl2{my, mz};
+    EF2(FN, I, NFN, UEDEN, e) = dbl2{AVN(UEDEN, o.f_E), AVN(UEINT, o.f_e)};
+    EF2(FN, I, NFN, UTEMP, e) = dbl2{0.0, o.g_p};  // UTEMP, FN_GP
+#undef AVN
+    return v;
+}
+
+// species flux of species pair pr with its artificial viscosity (the
+// corrected species x0 of the minus side, x1 of the plus side), added to
+// the running sum in species order
+__device__ __forceinline__ void final_species_pair(const FinalAV &v, const RiemannH &o, const SpeciesPairs &XU,
+                                                   int pr, const double *x0, const double *x1, double *fs,
+                                                   double &sum)
+{
+    const dbl2 ur = SP2(XU, pr, v.cr), ul = SP2(XU, pr, v.cl);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int n = 2 * pr + h;
+        if (n >= NSPEC) break;
+        const double f = o.f_rho * (o.fpl * x0[h] + o.fmr * x1[h]);
+        fs[n] = f + v.dxd * (v.div1 * ((h == 0 ? ur.x : ur.y) - (h == 0 ? ul.x : ul.y)));
+        sum = sum + fs[n];
+    }
+}
+
+// normalize_species_fluxes (Source/Castro_hydro_nd.F90:248-297) and the
+// species and Godunov-u stores of face e
+__device__ __forceinline__ void final_species_store(const SpeciesPairs &XN, double *FN, const FInst &I, idx_t e,
+                                                    const double *fs, double sum, double frho, double g_un)
+{
+    const double fac = (sum != 0.0) ? frho / sum : 1.0;
+#pragma unroll
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr)
+        SP2(XN, pr, e) = dbl2{fs[2 * pr] * fac, 2 * pr + 1 < NSPEC ? fs[2 * pr + 1] * fac : 0.0};
+    EF(FN, I, NFN, FN_GU, e) = g_un;
+}
+
 // trans2 (Source/trans.F90:250-505) + final flux + apply_av
 // (Source/Castro_hydro_nd.F90:174-245) + normalize_species_fluxes (:248-297)
 // on the d-faces of the valid region.
@@ -415,36 +752,12 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
     }
     RiemannH o;
     face_riemann_h(x, I, d, i, j, k, side[0], side[1], o);
-
-    // artificial viscosity with the reference's corner-node pattern
-    double div1;
-#define DVN(a, b, cc) EA(x.DV, I, 1, 0, eidx(I, a, b, cc))
-    if (d == 0) div1 = FOURTH * (DVN(i, j, k) + DVN(i, j + 1, k) + DVN(i, j, k + 1) + DVN(i, j + 1, k));
-    else if (d == 1) div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j, k + 1) + DVN(i + 1, j, k));
-    else div1 = FOURTH * (DVN(i, j, k) + DVN(i + 1, j, k) + DVN(i, j + 1, k) + DVN(i + 1, j + 1, k));
-#undef DVN
-    div1 = 0.1 * fmin(0.0, div1);
-    const idx_t cr = cidx(I, i, j, k), cl = cidx(I, i - ed[0], j - ed[1], k - ed[2]);
-    const double dxd = I.dx[d];
-#define AVN(n, fval) ((fval) + dxd * (div1 * (CA(x.U1, I, NU1, n, cr) - CA(x.U1, I, NU1, n, cl))))
-    const int mn = UMX + d, mt = d == 0 ? UMY : UMX, mtt = d == 2 ? UMY : UMZ;
     double *FN = x.FN[d];
     const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
                        X2 = species_pairs(F2, I, NFF, FF_X), X3 = species_pairs(F3, I, NFF, FF_X),
                        XN = species_pairs(FN, I, NFN, FN_X);
     const SpeciesPairs XU = cell_species(x.U1X, I);
-    const double frho = AVN(URHO, o.f_rho);
-    {
-        // momenta back in absolute order (mn = UMX + d, mt, mtt as above)
-        const double vmn = AVN(mn, o.f_mn), vmt = AVN(mt, o.f_mt), vmtt = AVN(mtt, o.f_mtt);
-        const double mx = d == 0 ? vmn : vmt;
-        const double my = d == 0 ? vmt : (d == 1 ? vmn : vmtt);
-        const double mz = d == 2 ? vmn : vmtt;
-        EF2(FN, I, NFN, URHO, e) = dbl2{frho, mx};
-        EF2(FN, I, NFN, UMY, e) = dbl2{my, mz};
-        EF2(FN, I, NFN, UEDEN, e) = dbl2{AVN(UEDEN, o.f_E), AVN(UEINT, o.f_e)};
-        EF2(FN, I, NFN, UTEMP, e) = dbl2{0.0, o.g_p};  // UTEMP, FN_GP
-    }
+    const FinalAV v = final_hydro(x, I, d, i, j, k, e, o);
     double fs[NSPEC], sum = 0.0;
     const bool useL = o.fpl != 0.0, useR = o.fmr != 0.0;
 #pragma unroll
@@ -465,22 +778,139 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
             x1[0] = trans2_species(rho[1], s.x, a.x, b.x, c3.x, d3.x, h2, h3, rrnew[1]);
             if (two) x1[1] = trans2_species(rho[1], s.y, a.y, b.y, c3.y, d3.y, h2, h3, rrnew[1]);
         }
-        const dbl2 ur = SP2(XU, pr, cr), ul = SP2(XU, pr, cl);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int n = 2 * pr + h;
-            if (n >= NSPEC) break;
-            const double f = o.f_rho * (o.fpl * x0[h] + o.fmr * x1[h]);
-            fs[n] = f + dxd * (div1 * ((h == 0 ? ur.x : ur.y) - (h == 0 ? ul.x : ul.y)));
-            sum = sum + fs[n];
+        final_species_pair(v, o, XU, pr, x0, x1, fs, sum);
+    }
+    final_species_store(XN, FN, I, e, fs, sum, v.frho, o.g_un);
+}
+
+// K_final by cells: the thread of cell c (bricks elongated along d,
+// overlapping by one cell, as K_tflux) applies trans2 to both d-states of
+// its cell, the plus state of d-face c and the minus state of d-face c + 1,
+// with the four transverse fluxes on its cell's t1- and t2-faces, which it
+// reads once for both (K_final reads them once per face side).  The minus
+// side goes through LDS to the thread above, which solves the Riemann
+// problem on its d-face and applies the artificial viscosity and the
+// species normalisation as K_final does; species sides as in K_trans1c.
+__global__ __launch_bounds__(256) void K_finalc(Plan3 p3, Ctx x)
+{
+    // the minus sides' hydro part, then (after the barrier that ends its
+    // reads) their species in the same memory
+    __shared__ union {
+        FaceH h[256];
+        dbl2 s[NSPEC_PAIRS][256];
+    } lds;
+    __shared__ unsigned char lds_f[256];
+    FaceH(&lds_h)[256] = lds.h;
+    dbl2(&lds_s)[NSPEC_PAIRS][256] = lds.s;
+    int lb = logical_block(), d;
+    if (gridDim.y > 1) {
+        d = blockIdx.y;
+    } else {
+        d = lb % 3;
+        lb /= 3;
+    }
+    const Plan p = d == 0 ? p3.p[0] : (d == 1 ? p3.p[1] : p3.p[2]);
+    if (lb >= (d == 0 ? p3.nb[0] : (d == 1 ? p3.nb[1] : p3.nb[2]))) return;  // the whole block
+    int m, i, j, k, l;
+    const bool ok = plan_cell_ovl(p, d, lb, m, i, j, k, l);
+    const FInst &I = p.inst[m];
+    const int tid = threadIdx.x;
+    const int bd = d == 0 ? p.bx : (d == 1 ? p.by : p.bz);
+    const int st = d == 0 ? 1 : (d == 1 ? p.bx : p.bx * p.by);
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int cd = d == 0 ? i : (d == 1 ? j : k), c1 = t1 == 0 ? i : j, c2 = t2 == 1 ? j : k;
+    // d-faces lo_index..n_d (the slab's top in z), transverse cells lo_index..hi_cell
+    const int dmax = d == 2 ? I.zlo + I.zn : I.n[d];
+    const bool act = ok && c1 >= lo_index(I, t1) && c1 <= hi_cell(I, t1) && c2 >= lo_index(I, t2) &&
+                     c2 <= hi_cell(I, t2);
+    const bool has_p = act && l > 0 && cd >= lo_index(I, d);  // d-face c (the plus side)
+    const bool has_m = act && l + 1 < bd && cd + 1 <= dmax;   // d-face c + 1 (the minus side)
+    const idx_t ep = eidx(I, i, j, k), em = eidx(I, i + (d == 0), j + (d == 1), k + (d == 2));
+    const idx_t l2 = ep, r2 = eidx(I, i + (t1 == 0), j + (t1 == 1), k), r3 = eidx(I, i, j + (t2 == 1), k + (t2 == 2));
+    const double dt = x.dt_cur[m];
+    const double h2 = 0.5 * dt / I.dx[t1], h3 = 0.5 * dt / I.dx[t2];
+    double *F2 = x.FT[pair_index(t1, t2)], *F3 = x.FT[pair_index(t2, t1)], *FN = x.FN[d];
+    const SpeciesPairs XL = species_pairs(x.SM[d], I, NFS, FS_X), XR = species_pairs(x.SP[d], I, NFS, FS_X),
+                       X2 = species_pairs(F2, I, NFF, FF_X), X3 = species_pairs(F3, I, NFF, FF_X),
+                       XN = species_pairs(FN, I, NFN, FN_X);
+    const SpeciesPairs XU = cell_species(x.U1X, I);
+    FaceH side_p;
+    double rho_p = 0.0, rrnew_p = 0.0, rho_m = 0.0, rrnew_m = 0.0;
+    if (has_p || has_m) {
+        double f2r[UEDEN + 1], f2l[UEDEN + 1], f3r[UEDEN + 1], f3l[UEDEN + 1], pg2p, ug2p, pg2m, ug2m, pg3p, ug3p,
+            pg3m, ug3m;
+        load_tflux_h(F2, I, r2, f2r, pg2p, ug2p);
+        load_tflux_h(F2, I, l2, f2l, pg2m, ug2m);
+        load_tflux_h(F3, I, r3, f3r, pg3p, ug3p);
+        load_tflux_h(F3, I, l2, f3l, pg3m, ug3m);
+        const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, i, j, k));
+        if (has_m) {
+            QInH lq;
+            load_qin_h(x.SM[d], I, em, lq);
+            const QStateH o = trans2_hydro(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m,
+                                           gamc, h2, h3);
+            rho_m = lq.rho;
+            rrnew_m = lq.rho - h2 * (f2r[URHO] - f2l[URHO]) - h3 * (f3r[URHO] - f3l[URHO]);
+            lds_h[tid] = to_face(o, d);
+        }
+        if (has_p) {
+            QInH lq;
+            load_qin_h(x.SP[d], I, ep, lq);
+            const QStateH o = trans2_hydro(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m,
+                                           gamc, h2, h3);
+            rho_p = lq.rho;
+            rrnew_p = lq.rho - h2 * (f2r[URHO] - f2l[URHO]) - h3 * (f3r[URHO] - f3l[URHO]);
+            side_p = to_face(o, d);
         }
     }
-#undef AVN
-    const double fac = (sum != 0.0) ? frho / sum : 1.0;
+    __syncthreads();
+    RiemannH o;
+    FinalAV v;
+    bool useL = false, useR = false;
+    if (has_p) {
+        face_riemann_h(x, I, d, i, j, k, lds_h[below_of(tid, st)], side_p, o);
+        v = final_hydro(x, I, d, i, j, k, ep, o);
+        useL = o.fpl != 0.0;
+        useR = o.fmr != 0.0;
+    }
+    lds_f[tid] = useL;
+    __syncthreads();
+    const bool need_m = has_m && lds_f[tid + st];  // the face above reads this minus side
+    // every pair's corrected species first (the minus side to LDS, the plus
+    // side kept), so that their loads are in flight together; one barrier
+    dbl2 xpv[NSPEC_PAIRS];
 #pragma unroll
-    for (int pr = 0; pr < NSPEC_PAIRS; ++pr)
-        SP2(XN, pr, e) = dbl2{fs[2 * pr] * fac, 2 * pr + 1 < NSPEC ? fs[2 * pr + 1] * fac : 0.0};
-    EF(FN, I, NFN, FN_GU, e) = o.g_un;
+    for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+        const bool two = 2 * pr + 1 < NSPEC;  // the last pair holds one species
+        double xm[2] = {0.0, 0.0}, xp[2] = {0.0, 0.0};
+        if (need_m || useR) {
+            const dbl2 a = SP2(X2, pr, r2), b = SP2(X2, pr, l2);
+            const dbl2 c3 = SP2(X3, pr, r3), d3 = SP2(X3, pr, l2);
+            if (need_m) {
+                const dbl2 s = SP2(XL, pr, em);
+                xm[0] = trans2_species(rho_m, s.x, a.x, b.x, c3.x, d3.x, h2, h3, rrnew_m);
+                if (two) xm[1] = trans2_species(rho_m, s.y, a.y, b.y, c3.y, d3.y, h2, h3, rrnew_m);
+            }
+            if (useR) {
+                const dbl2 s = SP2(XR, pr, ep);
+                xp[0] = trans2_species(rho_p, s.x, a.x, b.x, c3.x, d3.x, h2, h3, rrnew_p);
+                if (two) xp[1] = trans2_species(rho_p, s.y, a.y, b.y, c3.y, d3.y, h2, h3, rrnew_p);
+            }
+        }
+        lds_s[pr][tid] = dbl2{xm[0], xm[1]};
+        xpv[pr] = dbl2{xp[0], xp[1]};
+    }
+    __syncthreads();
+    if (has_p) {
+        double fs[NSPEC], sum = 0.0;
+#pragma unroll
+        for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
+            const dbl2 x0v = useL ? lds_s[pr][below_of(tid, st)] : dbl2{0.0, 0.0};
+            const double x0[2] = {x0v.x, x0v.y}, x1[2] = {xpv[pr].x, xpv[pr].y};
+            final_species_pair(v, o, XU, pr, x0, x1, fs, sum);
+        }
+        final_species_store(XN, FN, I, ep, fs, sum, v.frho, o.g_un);
+    }
 }
 
 // fill_hydro_source (Source/Castro_hydro_nd.F90:344-441), Saxpy and the two
@@ -607,38 +1037,81 @@ __global__ __launch_bounds__(256) void K_arena(const ArenaBox *boxes, int nbox, 
 
 // ------------------------------------------------------------------ host side
 // index range of each stage: cells [0,n), faces [0,n], or the ring [-1,n]
-static const int stage_shift[NSTAGE] = {0, 0, 1, 1, 1, 1, 0, 0};
-static const int stage_add[NSTAGE] = {0, 0, 2, 2, 2, 2, 1, 0};
+static const int stage_shift[NSTAGE] = {0, 0, 1, 1, 1, 1, 0, 0, 1, 1, 1};
+static const int stage_add[NSTAGE] = {0, 0, 2, 2, 2, 2, 1, 0, 2, 2, 2};
+// sub-launches (directions or pairs) of each stage
+static const int stage_ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1, 3, 6, 3};
+// stages whose bricks overlap along their direction (plan_cell_ovl)
+static bool overlap_stage(int stage) { return stage == ST_TFLUX || stage == ST_TRANS1C || stage == ST_FINALC; }
+
+// the stages one slab of a step runs, in order (after K_prim)
+static std::vector<int> step_stages(const FusedLevel *F)
+{
+    std::vector<int> v = {ST_FLAT};
+    if (F->variant & FV_SPLIT_TRACE) {
+        v.push_back(ST_TRACE);
+        v.push_back(ST_FLUX1);
+    } else {
+        v.push_back(ST_TFLUX);
+    }
+    v.push_back(F->variant & FV_FACE_TRANS1 ? ST_TRANS1 : ST_TRANS1C);
+    v.push_back(F->variant & FV_CELL_FINAL ? ST_FINALC : ST_FINAL);
+    v.push_back(ST_UPDATE);
+    return v;
+}
 
 // the cell stages before and after the face stages of a step (estdt,
 // K_prim) run over whole instances; the face stages and K_update run slab
 // by slab
 static bool whole_instance_stage(int stage) { return stage == ST_EST || stage == ST_PRIM; }
 
-Plan get_plan(FusedLevel *F, int stage, int slab)
+// brick of an overlapping stage's direction d for its tiling (W, L, X)
+static KernelTiling ovl_brick(const KernelTiling &t, int d)
 {
-    if (whole_instance_stage(stage)) slab = -1;
-    const KernelTiling t = F->tiling[stage];
-    const int sh = stage_shift[stage], add = stage_add[stage];
+    if (d == 0) return KernelTiling{t.bz, t.bx * t.by / t.bz, 1};
+    if (d == 1) return KernelTiling{t.bx, t.by, 1};
+    return KernelTiling{t.bx, 1, t.by};
+}
+
+// an overlapping stage's tiling is valid when every direction's brick spans
+// at least two cells along its direction and X divides the thread count
+static bool ovl_tiling_ok(const KernelTiling &t)
+{
+    const int nt = t.bx * t.by;
+    return t.bx > 0 && t.by >= 2 && t.bz >= 2 && nt <= 256 && nt % 64 == 0 && nt % t.bz == 0;
+}
+
+// block table of a brick tiling t over the range (shift, add) of a slab;
+// ovl >= 0: the bricks overlap along that direction (plan_cell_ovl)
+static Plan plan_for(FusedLevel *F, const KernelTiling &t, int sh, int add, int slab, int ovl)
+{
     const FInst *tab = slab < 0 ? F->inst : F->sinst[slab];
     for (auto &pb : F->plans)
         if (pb.t.bx == t.bx && pb.t.by == t.by && pb.t.bz == t.bz && pb.shift == sh && pb.add == add &&
-            pb.slab == slab)
+            pb.slab == slab && pb.ovl == ovl)
             return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
     FusedLevel::PlanBuf pb;
     pb.t = t;
     pb.shift = sh;
     pb.add = add;
     pb.slab = slab;
+    pb.ovl = ovl;
     const std::vector<FInst> &hs = slab < 0 ? F->hinst : F->hsinst[slab];
     std::vector<int> b0(F->ninst + 1);
+    const int bb[3] = {t.bx, t.by, t.bz};
     int acc = 0;
     for (int m = 0; m < F->ninst; ++m) {
         b0[m] = acc;
         const FInst &I = hs[m];
-        const int e0 = I.n[0] + add, e1 = I.n[1] + add, e2 = I.zn + add;
+        const int e[3] = {I.n[0] + add, I.n[1] + add, I.zn + add};
         // an instance with no cells in this slab launches no bricks
-        if (I.zn > 0) acc += ((e0 + t.bx - 1) / t.bx) * ((e1 + t.by - 1) / t.by) * ((e2 + t.bz - 1) / t.bz);
+        if (I.zn <= 0) continue;
+        int nb = 1;
+        for (int a = 0; a < 3; ++a) {
+            const int st = bb[a] - (a == ovl);  // brick stride (plan_cell_ovl)
+            nb *= (e[a] - (a == ovl) + st - 1) / st;
+        }
+        acc += nb;
     }
     b0[F->ninst] = acc;
     pb.nblocks = acc;
@@ -647,6 +1120,23 @@ Plan get_plan(FusedLevel *F, int stage, int slab)
         return Plan{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
     F->plans.push_back(pb);
     return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+}
+
+Plan get_plan(FusedLevel *F, int stage, int slab)
+{
+    if (whole_instance_stage(stage)) slab = -1;
+    return plan_for(F, F->tiling[stage], stage_shift[stage], stage_add[stage], slab, -1);
+}
+
+// the three direction plans of an overlapping stage
+static Plan3 get_plan3(FusedLevel *F, int stage, int slab)
+{
+    Plan3 q;
+    for (int d = 0; d < 3; ++d) {
+        q.p[d] = plan_for(F, ovl_brick(F->tiling[stage], d), stage_shift[stage], stage_add[stage], slab, d);
+        q.nb[d] = plan_blocks(F, q.p[d]);
+    }
+    return q;
 }
 
 int plan_blocks(FusedLevel *F, const Plan &p)
@@ -665,6 +1155,18 @@ namespace mc {
 static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1, int slab = 0)
 {
     F->ctx.tab = eos_table();
+    if (overlap_stage(stage)) {
+        // the directions / pairs interleaved block by block; one with fewer
+        // bricks than the largest leaves its last blocks idle
+        const Plan3 q = get_plan3(F, stage, slab);
+        const int nb = std::max(q.nb[0], std::max(q.nb[1], q.nb[2]));
+        if (nb <= 0) return;
+        const KernelTiling &t = F->tiling[stage];
+        if (stage == ST_TFLUX) hipLaunchKernelGGL(K_tflux, dim3(3 * nb), dim3(t.bx * t.by), 0, s, q, F->ctx);
+        else if (stage == ST_TRANS1C) hipLaunchKernelGGL(K_trans1c, dim3(6 * nb), dim3(t.bx * t.by), 0, s, q, F->ctx);
+        else hipLaunchKernelGGL(K_finalc, dim3(3 * nb), dim3(t.bx * t.by), 0, s, q, F->ctx);
+        return;
+    }
     Plan p = get_plan(F, stage, slab);
     const int nb = plan_blocks(F, p);
     const int nt = p.bx * p.by * p.bz;
@@ -749,6 +1251,7 @@ FusedLevel *fused_create_set(const std::vector<FusedInstanceSpec> &spec, int sla
     F->totQ = offQ;
     for (int s = 0; s < NSTAGE; ++s) F->tiling[s] = KernelTiling{64, 2, 2};
     F->tiling[ST_EST] = F->tiling[ST_PRIM] = F->tiling[ST_UPDATE] = KernelTiling{64, 4, 1};
+    F->tiling[ST_TFLUX] = F->tiling[ST_TRANS1C] = F->tiling[ST_FINALC] = KernelTiling{16, 16, 64};
     bool ok = !check_hip(hipMalloc(&F->inst, sizeof(FInst) * F->ninst), "hipMalloc(inst)") &&
               !check_hip(hipMemcpy(F->inst, F->hinst.data(), sizeof(FInst) * F->ninst, hipMemcpyHostToDevice), "h2d");
     F->sinst.assign(F->nslab, nullptr);
@@ -1034,12 +1537,7 @@ int fused_estimate(FusedLevel *F, hipStream_t s)
 // the face stages and the update of one z-slab
 static void enqueue_slab(FusedLevel *F, int sl, hipStream_t s)
 {
-    launch_stage(F, ST_FLAT, s, 1, sl);
-    launch_stage(F, ST_TRACE, s, 3, sl);
-    launch_stage(F, ST_FLUX1, s, 3, sl);
-    launch_stage(F, ST_TRANS1, s, 6, sl);
-    launch_stage(F, ST_FINAL, s, 3, sl);
-    launch_stage(F, ST_UPDATE, s, 1, sl);
+    for (int st : step_stages(F)) launch_stage(F, st, s, stage_ny[st], sl);
 }
 
 // enqueue one coarse step of every instance.  K_prim runs over whole
@@ -1067,13 +1565,14 @@ int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s
         return -1;
     }
     if (!F->have_est && fused_estimate(F, s)) return -1;
-    static const int slab_order[6] = {ST_FLAT, ST_TRACE, ST_FLUX1, ST_TRANS1, ST_FINAL, ST_UPDATE};
-    static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    const std::vector<int> slab_order = step_stages(F);
+    const int nso = (int)slab_order.size();
+    const int *ny = stage_ny;
     std::vector<int> stage_of;  // stage of each timed launch of one step (NSTAGE: dt control)
     stage_of.push_back(NSTAGE);
     stage_of.push_back(ST_PRIM);
     for (int sl = 0; sl < F->nslab; ++sl)
-        for (int q = 0; q < 6; ++q) stage_of.push_back(slab_order[q]);
+        for (int q = 0; q < nso; ++q) stage_of.push_back(slab_order[q]);
     const size_t per = stage_of.size();
     std::vector<hipEvent_t> ev(2 * per * (size_t)nsteps, nullptr);
     int rc = 0;
@@ -1092,7 +1591,7 @@ int fused_steps_timed(FusedLevel *F, int nsteps, double stop_time, hipStream_t s
         (void)hipEventRecord(E[2 * q + 1], s);
         ++q;
         for (int sl = 0; sl < F->nslab; ++sl)
-            for (int k = 0; k < 6; ++k, ++q) {
+            for (int k = 0; k < nso; ++k, ++q) {
                 (void)hipEventRecord(E[2 * q], s);
                 launch_stage(F, slab_order[k], s, ny[slab_order[k]], sl);
                 (void)hipEventRecord(E[2 * q + 1], s);
@@ -1138,8 +1637,11 @@ int fused_steps(FusedLevel *F, int nsteps, double stop_time, hipStream_t s, bool
         F->gexec = nullptr;
         F->graph = nullptr;
         // make sure every plan table exists before capture
-        for (int st = 0; st < NSTAGE; ++st)
-            for (int sl = 0; sl < F->nslab; ++sl) (void)get_plan(F, st, sl);
+        for (int sl = 0; sl < F->nslab; ++sl)
+            for (int st = 0; st < NSTAGE; ++st) {
+                if (overlap_stage(st)) (void)get_plan3(F, st, sl);
+                else (void)get_plan(F, st, sl);
+            }
         if (check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) return -1;
         enqueue_step(F, stop_time, s);
         if (check_hip(hipStreamEndCapture(s, &F->graph), "end capture")) return -1;
@@ -1185,10 +1687,8 @@ int fused_recompute_dt(FusedLevel *F, double dt, hipStream_t s)
     fill(F->ctx.dt_cur, F->ninst, dt, s);
     launch_stage(F, ST_PRIM, s);
     launch_stage(F, ST_FLAT, s);
-    launch_stage(F, ST_TRACE, s, 3);
-    launch_stage(F, ST_FLUX1, s, 3);
-    launch_stage(F, ST_TRANS1, s, 6);
-    launch_stage(F, ST_FINAL, s, 3);
+    for (int st : step_stages(F))
+        if (st != ST_UPDATE) launch_stage(F, st, s, stage_ny[st]);
     return check_hip(hipGetLastError(), "fused_recompute_dt");
 }
 
@@ -1221,9 +1721,27 @@ int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s)
     return check_hip(hipStreamSynchronize(s), "est sync");
 }
 
+void fused_set_variant(FusedLevel *F, int flags)
+{
+    F->variant = flags & FV_ALL;
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    F->gexec = nullptr;
+    F->graph = nullptr;
+}
+
+int fused_variant(const FusedLevel *F) { return F->variant; }
+
+bool fused_tiling_ok(int stage, int bx, int by, int bz)
+{
+    if (stage < 0 || stage >= NSTAGE) return false;
+    if (overlap_stage(stage)) return ovl_tiling_ok(KernelTiling{bx, by, bz});
+    return bx > 0 && by > 0 && bz > 0 && (bx * by * bz) % 64 == 0 && bx * by * bz <= 256;
+}
+
 void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz)
 {
-    if (stage < 0 || stage >= NSTAGE) return;
+    if (!fused_tiling_ok(stage, bx, by, bz)) return;
     F->tiling[stage] = KernelTiling{bx, by, bz};
     if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
     if (F->graph) (void)hipGraphDestroy(F->graph);
@@ -1244,7 +1762,7 @@ int fused_run_stage(FusedLevel *F, int stage, hipStream_t s)
         set_error("fused engine: EOS table not initialised (call eos_init or mc_eos_init_*)");
         return -1;
     }
-    static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    const int *ny = stage_ny;
     if (stage < 0 || stage >= NSTAGE) {
         set_error("fused_run_stage: no such stage");
         return -1;
@@ -1261,6 +1779,12 @@ static const KernelTiling tune_cand[] = {
     {64, 1, 1}, {32, 2, 1}, {16, 4, 1}, {8, 8, 1},  {16, 2, 2}, {128, 1, 1}, {64, 2, 1}, {32, 4, 1},
     {16, 8, 1}, {32, 2, 2}, {16, 4, 2}, {8, 8, 2},  {64, 4, 1}, {64, 2, 2},  {32, 8, 1}, {32, 4, 2},
     {16, 16, 1}, {16, 8, 2}, {16, 4, 4}, {8, 8, 4}, {128, 2, 1}};
+// candidates (W, L, X) of the overlapping stages: y bricks (W, L, 1), z
+// bricks (W, 1, L), x bricks (X, W*L/X, 1); longer bricks along the
+// direction recompute fewer overlap layers, wider ones read longer rows
+static const KernelTiling tflux_cand[] = {
+    {16, 16, 64}, {32, 8, 64}, {8, 32, 64}, {16, 16, 128}, {32, 8, 128}, {64, 4, 64}, {16, 8, 64},
+    {32, 4, 64}, {8, 16, 64}, {16, 16, 256}, {8, 8, 64}};
 
 // Tiling autotuner: times every candidate brick of every stage of the step
 // on the data one coarse step leaves behind and keeps the fastest.  The
@@ -1306,15 +1830,25 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) rc = check_hip(hipEventCreate(&e0), "hipEventCreate") || check_hip(hipEventCreate(&e1), "hipEventCreate");
     if (!rc) enqueue_step(F, -1.0, s);  // every intermediate array now holds data of a real step
-    static const int ny[NSTAGE] = {1, 1, 1, 3, 3, 6, 3, 1};
+    const int *ny = stage_ny;
     auto restore_state = [&]() {
         return check_hip(hipMemcpyAsync(x.U, saveU, nU, hipMemcpyDeviceToDevice, s), "autotune restore");
     };
-    for (int st = ST_PRIM; st <= ST_UPDATE && !rc; ++st) {
+    // the stages a step runs
+    std::vector<int> tuned = step_stages(F);
+    tuned.insert(tuned.begin(), ST_PRIM);
+    if (bricks)
+        for (int st = 0; st < NSTAGE; ++st) fused_get_tiling(F, st, bricks + 3 * st);
+    if (best_ms)
+        for (int st = 0; st < NSTAGE; ++st) best_ms[st] = 0.0;
+    for (int st : tuned) {
+        if (rc) break;
         const bool writes_state = st == ST_PRIM || st == ST_UPDATE;
         KernelTiling best = F->tiling[st];
         double tbest = 1e300;
-        for (const KernelTiling &c : tune_cand) {
+        std::vector<KernelTiling> cands(std::begin(tune_cand), std::end(tune_cand));
+        if (overlap_stage(st)) cands.assign(std::begin(tflux_cand), std::end(tflux_cand));
+        for (const KernelTiling &c : cands) {
             F->tiling[st] = c;
             float ms = 0.f;
             if (!writes_state) {
@@ -1353,12 +1887,6 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
         }
         if (best_ms) best_ms[st] = tbest;
     }
-    if (bricks) {
-        bricks[0] = F->tiling[ST_EST].bx;
-        bricks[1] = F->tiling[ST_EST].by;
-        bricks[2] = F->tiling[ST_EST].bz;
-    }
-    if (best_ms) best_ms[ST_EST] = 0.0;
     // restore the state, the clocks and the Newton-failure counter
     rc = restore_state() ||
          check_hip(hipMemcpyAsync(x.dt_cur, saveC, sizeof(double) * ni, hipMemcpyDeviceToDevice, s), "restore") ||

@@ -118,6 +118,39 @@ __device__ __forceinline__ bool plan_cell_sub(const Plan &p, int ny, int &sub, i
     return plan_cell_at(p, lb, m, i, j, k);
 }
 
+// Bricks that overlap along direction d: consecutive bricks along d start
+// bx-1 (by-1, bz-1) cells apart, so a brick's first layer along d (l = 0)
+// is the last layer of the brick below it, or the ring cell -1 for the
+// first brick.  Every cell of the range is a layer l >= 1 of exactly one
+// brick except that ring cell.  The brick count along d is
+// ceil((e - 1) / (b - 1)) for an extent of e cells (b >= 2).
+__device__ __forceinline__ bool plan_cell_ovl(const Plan &p, int d, int lb, int &m, int &i, int &j, int &k, int &l)
+{
+    m = find_inst(p, lb);
+    const FInst &I = p.inst[m];
+    const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.zn + p.add;
+    const int s0 = p.bx - (d == 0), s1 = p.by - (d == 1), s2 = p.bz - (d == 2);
+    const int nb0 = (e0 - (d == 0) + s0 - 1) / s0, nb1 = (e1 - (d == 1) + s1 - 1) / s1;
+    const int b = lb - p.blk0[m];
+    const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
+    const int t = threadIdx.x;
+    const int ti = t % p.bx, tj = (t / p.bx) % p.by, tk = t / (p.bx * p.by);
+    const int li = bi * s0 + ti, lj = bj * s1 + tj, lk = bk * s2 + tk;
+    l = d == 0 ? ti : (d == 1 ? tj : tk);
+    i = li - p.shift;
+    j = lj - p.shift;
+    k = lk - p.shift + I.zlo;
+    return li < e0 && lj < e1 && lk < e2;
+}
+
+// the three direction plans of an overlapping stage (K_tflux, K_trans1c,
+// K_finalc)
+// and their block counts
+struct Plan3 {
+    Plan p[3];
+    int nb[3];
+};
+
 __device__ __forceinline__ int bmap(int i, int n, int periodic)
 {
     return periodic ? ((i % n) + n) % n : min(max(i, 0), n - 1);
@@ -270,16 +303,24 @@ __device__ __forceinline__ void load_qin_h(double *S, const FInst &I, idx_t e, Q
     q.game = 0.0;
 }
 
-__device__ __forceinline__ FaceH to_face(const QStateH &o, int d)
+// The velocity components are passed by value: a conditional between two
+// fields of one struct selects an address, which keeps the struct in
+// scratch memory when the direction is a run-time value.
+__device__ __forceinline__ FaceH face_of(int d, double rho, double u, double v, double w, double p, double rhoe)
 {
     FaceH f;
-    f.rho = o.rho;
-    f.un = d == 0 ? o.u : (d == 1 ? o.v : o.w);
-    f.ut = d == 0 ? o.v : o.u;
-    f.utt = d == 2 ? o.v : o.w;
-    f.p = o.p;
-    f.rhoe = o.rhoe;
+    f.rho = rho;
+    f.un = d == 0 ? u : (d == 1 ? v : w);
+    f.ut = d == 0 ? v : u;
+    f.utt = d == 2 ? v : w;
+    f.p = p;
+    f.rhoe = rhoe;
     return f;
+}
+
+__device__ __forceinline__ FaceH to_face(const QStateH &o, int d)
+{
+    return face_of(d, o.rho, o.u, o.v, o.w, o.p, o.rhoe);
 }
 
 // Non-species part of a transverse-stage flux; momenta in absolute order.
@@ -351,11 +392,17 @@ struct FusedLevel {
     struct PlanBuf {
         KernelTiling t;
         int shift, add, slab;  // slab -1: the whole instance (cell stages)
+        int ovl;               // direction along which bricks overlap by one cell, -1 none
         int *blk0 = nullptr;
         int nblocks = 0;
     };
     std::vector<PlanBuf> plans;
-    KernelTiling tiling[8];  // per stage: est prim flat trace flux1 trans1 final update
+    // per stage: est prim flat trace flux1 trans1 final update tflux
+    // trans1c finalc; the last three are (W, L, X): bricks (W, L, 1) along
+    // y, (W, 1, L) along z and (X, W*L/X, 1) along x, each overlapping its
+    // neighbour along that direction by one cell
+    KernelTiling tiling[FST_N];
+    int variant = 0;  // FV_* flags
     // z-slabs of the face stages: planes per slab (0: the whole instance),
     // slab count, and one instance table per slab (zlo, zn set; slab 0 of a
     // whole-instance engine is `inst` itself)
@@ -376,7 +423,8 @@ struct FusedLevel {
 };
 
 enum { ST_EST = FST_EST, ST_PRIM = FST_PRIM, ST_FLAT = FST_FLAT, ST_TRACE = FST_TRACE, ST_FLUX1 = FST_FLUX1,
-       ST_TRANS1 = FST_TRANS1, ST_FINAL = FST_FINAL, ST_UPDATE = FST_UPDATE, NSTAGE = FST_N };
+       ST_TRANS1 = FST_TRANS1, ST_FINAL = FST_FINAL, ST_UPDATE = FST_UPDATE, ST_TFLUX = FST_TFLUX,
+       ST_TRANS1C = FST_TRANS1C, ST_FINALC = FST_FINALC, NSTAGE = FST_N };
 // launch plan of a stage under its current tiling (block table cached);
 // slab >= 0 selects a z-slab of the face stages, -1 the whole instance
 Plan get_plan(FusedLevel *F, int stage, int slab = 0);

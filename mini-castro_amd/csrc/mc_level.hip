@@ -167,6 +167,7 @@ struct mc_level {
     mc_flux_register *reg_crse = nullptr, *reg_fine = nullptr;
     std::vector<int> fine_index;
     int fused_slab_req = -1;  // z-slab planes of the fused engine: -1 chosen by memory, 0 whole level
+    int fused_variant = 0;  // kernel variant flags of the fused engine (FV_*)
 };
 
 static double *dalloc(size_t n, bool &ok)
@@ -376,6 +377,7 @@ static FusedLevel *level_engine(mc_level_t *L)
         slab = fused_auto_slab({sp});
     }
     L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx, slab);
+    if (L->fused) fused_set_variant(L->fused, L->fused_variant);
     return L->fused;
 }
 
@@ -436,9 +438,11 @@ int mc_level_fused_slab(mc_level_t *L)
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
 {
     if (!level_engine(L)) return -1;
-    if (stage < 0 || stage >= FST_N || bx <= 0 || by <= 0 || bz <= 0 || (bx * by * bz) % 64 ||
-        bx * by * bz > 256) {
-        set_error("mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256");
+    if (!fused_tiling_ok(stage, bx, by, bz)) {
+        set_error(stage == FST_TFLUX || stage == FST_TRANS1C || stage == FST_FINALC
+                      ? "mc_level_set_tiling: the tflux / trans1c / finalc tiling (W, L, X) needs L, X >= 2, W*L a "
+                        "multiple of 64 threads, at most 256, and X dividing W*L"
+                      : "mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256");
         return -1;
     }
     fused_set_tiling(L->fused, stage, bx, by, bz);
@@ -460,6 +464,17 @@ int mc_level_run_stage(mc_level_t *L, int stage)
         return -1;
     }
     return fused_run_stage(L->fused, stage, L->stream);
+}
+
+int mc_level_set_stage_variant(mc_level_t *L, int flags)
+{
+    if (flags & ~FV_ALL) {
+        set_error("mc_level_set_stage_variant: unknown flags");
+        return -1;
+    }
+    L->fused_variant = flags;
+    if (L->fused) fused_set_variant(L->fused, flags);
+    return 0;
 }
 
 int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms)

@@ -68,11 +68,8 @@ class Ensemble:
     def autotune(self, reps=3):
         """Pick the fastest brick of every stage (state unchanged).
         Returns {stage: ((bx, by, bz), ms_per_launch)}."""
-        b = (C.c_int * 24)()
-        ms = (C.c_double * 8)()
-        check(self.L.mc_ensemble_autotune(self.h, int(reps), b, ms), "mc_ensemble_autotune")
-        names = ("est", "prim", "flat", "trace", "flux1", "trans1", "final", "update")
-        return {names[s]: ((b[3 * s], b[3 * s + 1], b[3 * s + 2]), ms[s]) for s in range(1, 8)}
+        from .level import Level, _autotune
+        return _autotune(self.L.mc_ensemble_autotune, self.h, reps, Level.STAGES, "mc_ensemble_autotune")
 
     def zones(self):
         return sum(int(np.prod(d["n_cell"])) for d in self.inst)

@@ -97,15 +97,29 @@ class Level:
         step is replayed from a captured hipGraph."""
         check(self.L.mc_level_run(self.h, int(nsteps), stop_time, 1 if use_graph else 0), "mc_level_run")
 
-    STAGES = ("est", "prim", "flat", "trace", "flux1", "trans1", "final", "update", "dt")
+    # fused stages: "tflux" is "trace" and "flux1" in one launch, "trans1c"
+    # and "finalc" are "trans1" and "final" by cells (which ones a step runs:
+    # set_stage_variant); "dt" the time-step control kernels
+    STAGES = ("est", "prim", "flat", "trace", "flux1", "trans1", "final", "update", "tflux", "trans1c", "finalc",
+              "dt")
+    SPLIT_TRACE, FACE_TRANS1, CELL_FINAL = 1, 2, 4
 
     def run_timed(self, nsteps, stop_time=-1.0):
         """Fused mode: nsteps eager steps, HIP events around each stage launch.
-        Returns {stage: (total_ms, launches)}."""
-        ms = (C.c_double * 9)()
-        nl = (C.c_int * 9)()
+        Returns {stage: (total_ms, launches)}; a stage the step does not run
+        has 0 launches."""
+        n = len(self.STAGES)
+        ms = (C.c_double * n)()
+        nl = (C.c_int * n)()
         check(self.L.mc_level_run_timed(self.h, int(nsteps), stop_time, ms, nl), "mc_level_run_timed")
-        return {self.STAGES[i]: (ms[i], nl[i]) for i in range(9)}
+        return {self.STAGES[i]: (ms[i], nl[i]) for i in range(n)}
+
+    def set_stage_variant(self, flags):
+        """Fused mode: kernel variants, bit flags (0, the default: tflux,
+        trans1c, final).  SPLIT_TRACE runs trace + flux1 instead of tflux,
+        FACE_TRANS1 trans1 instead of trans1c, CELL_FINAL finalc instead of
+        final; results are identical."""
+        check(self.L.mc_level_set_stage_variant(self.h, int(flags)), "mc_level_set_stage_variant")
 
     def set_fused_slab(self, planes):
         """z-slab planes of the fused engine's face arrays: -1 chosen by free
@@ -133,10 +147,7 @@ class Level:
     def autotune(self, reps=3):
         """Fused mode: pick the fastest brick of every stage (state unchanged).
         Returns {stage: ((bx, by, bz), ms_per_launch)}."""
-        b = (C.c_int * 24)()
-        ms = (C.c_double * 8)()
-        check(self.L.mc_level_autotune(self.h, int(reps), b, ms), "mc_level_autotune")
-        return {self.STAGES[s]: ((b[3 * s], b[3 * s + 1], b[3 * s + 2]), ms[s]) for s in range(1, 8)}
+        return _autotune(self.L.mc_level_autotune, self.h, reps, self.STAGES, "mc_level_autotune")
 
     def blast_radius(self):
         """(blast_mass, blast_radius_sum, max_density) of post_timestep's diagnostic."""
@@ -158,3 +169,13 @@ class Level:
     @property
     def dt(self):
         return self.L.mc_level_dt(self.h)
+
+
+def _autotune(fn, h, reps, names, what):
+    """{stage: ((bx, by, bz), ms_per_launch)} of the stages the autotuner
+    timed (the stages a step runs)."""
+    n = len(names) - 1
+    b = (C.c_int * (3 * n))()
+    ms = (C.c_double * n)()
+    check(fn(h, int(reps), b, ms), what)
+    return {names[s]: ((b[3 * s], b[3 * s + 1], b[3 * s + 2]), ms[s]) for s in range(1, n) if ms[s] > 0}

@@ -68,15 +68,27 @@ def test_fused_uneven_equals_plain_and_graph_replay():
 
 
 def test_tiling_does_not_change_results():
+    """Other bricks for every stage, with the default kernels (tflux,
+    trans1c, final) and the stages tflux and trans1c replace (trace +
+    flux1, trans1)."""
     cfg = small_cfg(n=24, bc=0, r_init=0.1)
     A = gpu_level(cfg)
     B = gpu_level(cfg)
+    C = gpu_level(cfg)
+    C.set_stage_variant(C.SPLIT_TRACE | C.FACE_TRANS1)
     for st, t in ((2, (16, 4, 4)), (3, (64, 1, 4)), (4, (8, 8, 4)), (5, (32, 2, 4)), (6, (4, 4, 16)),
-                  (1, (256, 1, 1)), (7, (8, 8, 2))):
+                  (1, (256, 1, 1)), (7, (8, 8, 2)), (8, (8, 16, 32)), (9, (32, 4, 128)), (10, (16, 4, 32))):
         B.set_tiling(st, *t)
+        C.set_tiling(st, *t)
     A.run(3, use_graph=False)
     B.run(3, use_graph=True)
+    C.run(3, use_graph=True)
     np.testing.assert_array_equal(A.state(), B.state())
+    np.testing.assert_array_equal(A.state(), C.state())
+    with pytest.raises(RuntimeError):
+        B.set_tiling(8, 16, 16, 3)      # X must divide W*L
+    with pytest.raises(RuntimeError):
+        B.set_tiling(8, 64, 1, 64)      # L >= 2
 
 
 def test_autotune_keeps_state_and_results():
@@ -91,9 +103,10 @@ def test_autotune_keeps_state_and_results():
     s0 = A.state()
     clock0 = (A.steps, A.dt, A.time)
     tuned = A.autotune(reps=1)
-    assert set(tuned) == {"prim", "flat", "trace", "flux1", "trans1", "final", "update"}
+    assert set(tuned) == {"prim", "flat", "tflux", "trans1c", "final", "update"}
     for st, ((bx, by, bz), ms) in tuned.items():
-        assert (bx * by * bz) % 64 == 0 and bx * by * bz <= 256 and ms > 0.0, (st, bx, by, bz, ms)
+        nt = bx * by if st in ("tflux", "trans1c", "finalc") else bx * by * bz
+        assert nt % 64 == 0 and nt <= 256 and ms > 0.0, (st, bx, by, bz, ms)
         assert A.get_tiling(A.STAGES.index(st)) == (bx, by, bz)
     np.testing.assert_array_equal(A.state(), s0)
     assert (A.steps, A.dt, A.time) == clock0

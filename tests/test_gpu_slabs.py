@@ -90,8 +90,9 @@ def test_autotune_and_timed_steps_in_slabs():
     np.testing.assert_array_equal(A.state(), s0)
     assert (A.steps, A.dt, A.time) == c0
     st = A.run_timed(2)
-    for k in ("prim", "flat", "trace", "flux1", "trans1", "final", "update"):
+    for k in ("prim", "flat", "tflux", "trans1c", "final", "update"):
         assert st[k][1] == (2 if k == "prim" else 2 * 4) and st[k][0] > 0, (k, st[k])
+    assert st["trace"][1] == st["flux1"][1] == st["trans1"][1] == st["finalc"][1] == 0
     B.run(2, use_graph=False)
     assert (A.steps, A.dt, A.time) == (B.steps, B.dt, B.time)
     np.testing.assert_array_equal(A.state(), B.state())

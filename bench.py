@@ -418,9 +418,10 @@ def main():
                     help="also time the CPU baseline step on one thread (about three minutes at 256^3)")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="only time the adjoint workload and print its JSON (for profiling)")
-    ap.add_argument("--stage-variant", type=int, default=0,
-                    help="kernel variant flags of the fused step (0: tflux, trans1c, final; 1: trace + flux1 "
-                         "instead of tflux; 2: trans1 instead of trans1c; 4: finalc instead of final)")
+    ap.add_argument("--stage-variant", type=int, default=-1,
+                    help="kernel variant flags of the fused step instead of the autotuner's choice (0: tflux, "
+                         "trans1c, final; 1: trace + flux1 instead of tflux; 2: trans1 instead of trans1c; "
+                         "4: finalc instead of final)")
     ap.add_argument("--tiling-json", default=None,
                     help="take the stage tiling from config.tiling of an earlier bench.py JSON line "
                          "instead of running the autotuner")
@@ -454,17 +455,20 @@ def main():
 
     # batched fused path
     L = mc.Level(n, cfg["boxes"], cfg["prob_lo"], cfg["prob_hi"], cfg["bc"], mc.MODE_FUSED)
-    L.set_stage_variant(a.stage_variant)
     L.init_data(**cfg)
     if a.tiling_json:
         with open(a.tiling_json) as f:
             prev = json.loads([ln for ln in f if ln.startswith("{")][-1])
         for st, b in prev["config"]["tiling"].items():
             L.set_tiling(L.STAGES.index(st), *b)
+        L.set_stage_variant(prev["config"].get("stage_variant", 0))
         tuned = None
     else:
-        tuned = L.autotune(reps=2)   # restores the state it times on
+        tuned = L.autotune(reps=2)   # restores the state it times on; picks the kernel variant
         log("autotuned")
+    if a.stage_variant >= 0:
+        L.set_stage_variant(a.stage_variant)
+    variant = L.stage_variant
     tiling = {L.STAGES[st]: list(L.get_tiling(st)) for st in range(1, len(L.STAGES) - 1)}
     L.run(a.warmup, use_graph=True)
     L.sync()
@@ -576,7 +580,9 @@ def main():
                       "graph_replay_ms_per_step": graph_ms, "steps_advanced": fused_steps_done,
                       "fom_zones_per_usec": value / 1e6,
                       "tiling": tiling, "tiling_source": "autotuner" if tuned else a.tiling_json,
-                      "stage_variant": a.stage_variant,
+                      "stage_variant": variant,
+                      "stage_variant_source": ("--stage-variant" if a.stage_variant >= 0 else
+                                               "autotuner" if tuned else a.tiling_json),
                       "stages": [st for st in STAGES_WITH_BYTES if stages[st][1]],
                       "per_box_ms_per_step": plain,
                       "batched_vs_per_box": (plain / ms_step) if plain else None,

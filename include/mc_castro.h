@@ -285,12 +285,13 @@ int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
    prim stage replaces the state by its clean and the update stage advances
    it, both in place; the other stages leave it unchanged. */
 int mc_level_run_stage(mc_level_t *L, int stage);
-/* Fused mode: time every candidate brick of every stage of the step (reps
-   launches each) on the data of one coarse step and keep the fastest; the
-   state and the clock are restored.  bricks (11 x 3 ints) and best_ms (11)
-   receive every stage's tiling and the tuned stages' time per launch (0 for
-   a stage not tuned: est, and the stages the step does not run); either may
-   be NULL. */
+/* Fused mode: time every candidate brick of every stage of every kernel
+   variant (reps launches each) on the data of one coarse step and keep the
+   fastest, then select the faster kernel of each alternative (the stage
+   variant below; mc_level_stage_variant reports it); the state and the clock
+   are restored.  bricks (11 x 3 ints) and best_ms (11) receive every stage's
+   tiling and the tuned stages' time per launch (0 for est, which is not
+   tuned); either may be NULL. */
 int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms);
 /* Fused mode: kernel variants of the step, bit flags (0, the default, runs
    the stages tflux, trans1c and final): MC_VARIANT_SPLIT_TRACE runs the
@@ -303,6 +304,7 @@ int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms);
 #define MC_VARIANT_FACE_TRANS1 2
 #define MC_VARIANT_CELL_FINAL 4
 int mc_level_set_stage_variant(mc_level_t *L, int flags);
+int mc_level_stage_variant(mc_level_t *L);  /* the flags in use, -1 on error */
 
 /* ---- AMR coupling of a coarse and a fine level ----
    The flux register of a fine level (AMReX FluxRegister as the reference

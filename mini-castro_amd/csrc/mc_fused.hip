@@ -1786,16 +1786,20 @@ static const KernelTiling tflux_cand[] = {
     {16, 16, 64}, {32, 8, 64}, {8, 32, 64}, {16, 16, 128}, {32, 8, 128}, {64, 4, 64}, {16, 8, 64},
     {32, 4, 64}, {8, 16, 64}, {16, 16, 256}, {8, 8, 64}};
 
-// Tiling autotuner: times every candidate brick of every stage of the step
-// on the data one coarse step leaves behind and keeps the fastest.  The
-// stages between K_prim and K_update read no array they write, so repeating
-// one of their launches recomputes the same outputs.  K_prim (its clean) and
-// K_update rewrite the state in place, so each of their timed launches starts
-// from the state the step started from (K_update's from K_prim's clean of
-// it, as in a step), restored by untimed copies.  The state, the clocks and
-// the EOS Newton-failure counter are restored afterwards, so results do not
-// change (tilings never change results).  bricks (8 x 3) and best_ms (8)
-// receive the chosen brick and its time per launch; either may be null.
+// Tiling autotuner: times every candidate brick of every stage of every
+// kernel variant on the data one coarse step leaves behind and keeps the
+// fastest, then picks the variant: K_tflux or K_trace + K_flux1, K_trans1c
+// or K_trans1, K_final or K_finalc, whichever is faster.  The stages between
+// K_prim and K_update read no array they write, so repeating one of their
+// launches recomputes the same outputs (and either kernel of an alternative
+// writes the same outputs).  K_prim (its clean) and K_update rewrite the
+// state in place, so each of their timed launches starts from the state the
+// step started from (K_update's from K_prim's clean of it, as in a step),
+// restored by untimed copies.  The state, the clocks and the EOS
+// Newton-failure counter are restored afterwards, so results do not change
+// (tilings and variants never change results).  bricks (NSTAGE x 3) and
+// best_ms (NSTAGE) receive every stage's brick and the tuned stages' time
+// per launch; either may be null.
 int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *best_ms)
 {
     if (!eos_ready()) {
@@ -1834,9 +1838,11 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
     auto restore_state = [&]() {
         return check_hip(hipMemcpyAsync(x.U, saveU, nU, hipMemcpyDeviceToDevice, s), "autotune restore");
     };
-    // the stages a step runs
-    std::vector<int> tuned = step_stages(F);
-    tuned.insert(tuned.begin(), ST_PRIM);
+    // every stage of every kernel variant; the faster kernel of each
+    // alternative is chosen below
+    const std::vector<int> tuned = {ST_PRIM,   ST_FLAT,  ST_TRACE,  ST_FLUX1,  ST_TFLUX,
+                                    ST_TRANS1, ST_TRANS1C, ST_FINAL, ST_FINALC, ST_UPDATE};
+    double tb[NSTAGE] = {0.0};
     if (bricks)
         for (int st = 0; st < NSTAGE; ++st) fused_get_tiling(F, st, bricks + 3 * st);
     if (best_ms)
@@ -1880,12 +1886,20 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
             }
         }
         F->tiling[st] = best;
+        tb[st] = tbest;
         if (bricks) {
             bricks[3 * st] = best.bx;
             bricks[3 * st + 1] = best.by;
             bricks[3 * st + 2] = best.bz;
         }
         if (best_ms) best_ms[st] = tbest;
+    }
+    if (!rc) {
+        int v = 0;
+        if (tb[ST_TRACE] + tb[ST_FLUX1] < tb[ST_TFLUX]) v |= FV_SPLIT_TRACE;
+        if (tb[ST_TRANS1] < tb[ST_TRANS1C]) v |= FV_FACE_TRANS1;
+        if (tb[ST_FINALC] < tb[ST_FINAL]) v |= FV_CELL_FINAL;
+        F->variant = v;
     }
     // restore the state, the clocks and the Newton-failure counter
     rc = restore_state() ||

@@ -123,7 +123,8 @@ __device__ __forceinline__ bool plan_cell_sub(const Plan &p, int ny, int &sub, i
 // is the last layer of the brick below it, or the ring cell -1 for the
 // first brick.  Every cell of the range is a layer l >= 1 of exactly one
 // brick except that ring cell.  The brick count along d is
-// ceil((e - 1) / (b - 1)) for an extent of e cells (b >= 2).
+// ceil((e - 1) / (b - 1)) for an extent of e cells (b >= 2).  Bricks are
+// numbered x fastest, then y, then z.
 __device__ __forceinline__ bool plan_cell_ovl(const Plan &p, int d, int lb, int &m, int &i, int &j, int &k, int &l)
 {
     m = find_inst(p, lb);

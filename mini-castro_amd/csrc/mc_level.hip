@@ -488,7 +488,15 @@ int mc_level_autotune(mc_level_t *L, int reps, int *bricks, double *best_ms)
         return -1;
     }
     if (!L->fused_ahead && sync_to_fused(L)) return -1;
-    return fused_autotune(L->fused, reps, L->stream, bricks, best_ms);
+    if (fused_autotune(L->fused, reps, L->stream, bricks, best_ms)) return -1;
+    L->fused_variant = fused_variant(L->fused);  // the variant the autotuner chose
+    return 0;
+}
+
+int mc_level_stage_variant(mc_level_t *L)
+{
+    if (!level_engine(L)) return -1;
+    return fused_variant(L->fused);
 }
 
 int mc_level_nbox(const mc_level_t *L) { return (int)L->boxes.size(); }

@@ -96,6 +96,7 @@ SIGS = [
     ("mc_level_run_stage", _I, [_vp, _I]),
     ("mc_level_autotune", _I, [_vp, _I, _ip, _dp]),
     ("mc_level_set_stage_variant", _I, [_vp, _I]),
+    ("mc_level_stage_variant", _I, [_vp]),
     ("mc_level_run_timed", _I, [_vp, _I, _D, _dp, _ip]),
     ("mc_level_blast_radius", _I, [_vp, _dp, _dp, _dp]),
     ("mc_flux_register_create", _vp, [_I, _ip, _ip, _ip, _I]),

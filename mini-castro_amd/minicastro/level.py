@@ -121,6 +121,14 @@ class Level:
         final; results are identical."""
         check(self.L.mc_level_set_stage_variant(self.h, int(flags)), "mc_level_set_stage_variant")
 
+    @property
+    def stage_variant(self):
+        """The kernel variant flags in use (the autotuner sets them)."""
+        v = self.L.mc_level_stage_variant(self.h)
+        if v < 0:
+            raise RuntimeError("mc_level_stage_variant: %s" % self.L.mc_last_error().decode())
+        return v
+
     def set_fused_slab(self, planes):
         """z-slab planes of the fused engine's face arrays: -1 chosen by free
         memory (the default), 0 the whole level, > 0 that many cell planes."""
@@ -145,8 +153,9 @@ class Level:
         check(self.L.mc_level_run_stage(self.h, int(stage)), "mc_level_run_stage")
 
     def autotune(self, reps=3):
-        """Fused mode: pick the fastest brick of every stage (state unchanged).
-        Returns {stage: ((bx, by, bz), ms_per_launch)}."""
+        """Fused mode: pick the fastest brick of every stage and the faster
+        kernel of each alternative (state unchanged; stage_variant reports
+        the choice).  Returns {stage: ((bx, by, bz), ms_per_launch)}."""
         return _autotune(self.L.mc_level_autotune, self.h, reps, self.STAGES, "mc_level_autotune")
 
     def blast_radius(self):

@@ -103,11 +103,18 @@ def test_autotune_keeps_state_and_results():
     s0 = A.state()
     clock0 = (A.steps, A.dt, A.time)
     tuned = A.autotune(reps=1)
-    assert set(tuned) == {"prim", "flat", "tflux", "trans1c", "final", "update"}
+    assert set(tuned) == {"prim", "flat", "trace", "flux1", "tflux", "trans1", "trans1c", "final", "finalc",
+                          "update"}
     for st, ((bx, by, bz), ms) in tuned.items():
         nt = bx * by if st in ("tflux", "trans1c", "finalc") else bx * by * bz
         assert nt % 64 == 0 and nt <= 256 and ms > 0.0, (st, bx, by, bz, ms)
         assert A.get_tiling(A.STAGES.index(st)) == (bx, by, bz)
+    # the faster kernel of each alternative
+    ms = {st: v[1] for st, v in tuned.items()}
+    want = ((A.SPLIT_TRACE if ms["trace"] + ms["flux1"] < ms["tflux"] else 0)
+            | (A.FACE_TRANS1 if ms["trans1"] < ms["trans1c"] else 0)
+            | (A.CELL_FINAL if ms["finalc"] < ms["final"] else 0))
+    assert A.stage_variant == want
     np.testing.assert_array_equal(A.state(), s0)
     assert (A.steps, A.dt, A.time) == clock0
     A.run(3, use_graph=True)

@@ -420,6 +420,9 @@ int mc_ensemble_run(mc_ensemble_t *E, int nsteps, double stop_time, int use_grap
 int mc_ensemble_clock(mc_ensemble_t *E, int m, double *dt, double *time, int *steps);
 /* as mc_level_autotune, over every instance */
 int mc_ensemble_autotune(mc_ensemble_t *E, int reps, int *bricks, double *best_ms);
+/* as mc_level_set_stage_variant / mc_level_stage_variant */
+int mc_ensemble_set_stage_variant(mc_ensemble_t *E, int flags);
+int mc_ensemble_stage_variant(const mc_ensemble_t *E);
 
 /* ---- discrete adjoint of the CTU step (one single-box instance) ----
    Forward: U0 -> U_N over nsteps steps of the fused path with a frozen dt

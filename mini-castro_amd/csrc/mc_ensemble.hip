@@ -153,6 +153,18 @@ int mc_ensemble_autotune(mc_ensemble_t *E, int reps, int *bricks, double *best_m
     return fused_autotune(E->F, reps, E->stream, bricks, best_ms);
 }
 
+int mc_ensemble_set_stage_variant(mc_ensemble_t *E, int flags)
+{
+    if (flags & ~FV_ALL) {
+        set_error("mc_ensemble_set_stage_variant: unknown flags");
+        return -1;
+    }
+    fused_set_variant(E->F, flags);
+    return 0;
+}
+
+int mc_ensemble_stage_variant(const mc_ensemble_t *E) { return fused_variant(E->F); }
+
 int mc_ensemble_clock(mc_ensemble_t *E, int m, double *dt, double *time, int *steps)
 {
     if (m < 0 || m >= E->ninst) return -1;

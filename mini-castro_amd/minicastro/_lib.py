@@ -127,6 +127,8 @@ SIGS = [
     ("mc_ensemble_run", _I, [_vp, _I, _D, _I]),
     ("mc_ensemble_clock", _I, [_vp, _I, _dp, _dp, _ip]),
     ("mc_ensemble_autotune", _I, [_vp, _I, _ip, _dp]),
+    ("mc_ensemble_set_stage_variant", _I, [_vp, _I]),
+    ("mc_ensemble_stage_variant", _I, [_vp]),
     ("mc_adjoint_create", _vp, [_ip, _dp, _dp, _I, _I]),
     ("mc_adjoint_destroy", None, [_vp]),
     ("mc_adjoint_forward", _I, [_vp, _vp, _vp, _I, _dp, _I]),

@@ -50,6 +50,15 @@ class Ensemble:
                                            ea.ctypes.data_as(_dp), ra.ctypes.data_as(_dp),
                                            int(self.inst[0].get("nsub", 10))), "mc_ensemble_init_data")
 
+    def set_stage_variant(self, flags):
+        """Kernel variant flags of the fused step (minicastro.Level.SPLIT_TRACE,
+        FACE_TRANS1, CELL_FINAL); results are identical."""
+        check(self.L.mc_ensemble_set_stage_variant(self.h, int(flags)), "mc_ensemble_set_stage_variant")
+
+    @property
+    def stage_variant(self):
+        return self.L.mc_ensemble_stage_variant(self.h)
+
     def run(self, nsteps, stop_time=-1.0, use_graph=True):
         check(self.L.mc_ensemble_run(self.h, int(nsteps), stop_time, 1 if use_graph else 0), "mc_ensemble_run")
 

@@ -52,15 +52,20 @@ def test_slabs_match_oracle(bc, slab):
     np.testing.assert_array_equal(G.state(), O.state())
 
 
-def test_ensemble_slabs_match_sequential_runs():
+@pytest.mark.parametrize("flags", [0, 4, 3])
+def test_ensemble_slabs_match_sequential_runs(flags):
     """Instances deeper and shallower than the slab (sides 16, 13, 9, 20 with
-    slabs of 6 planes): every instance equals its per-box run."""
+    slabs of 6 planes), with the default kernels, the final stage by cells,
+    and the split trace/flux with face-centred trans1: every instance equals
+    its per-box run."""
     insts = []
     for side, E, r, rho, bc in ((16, 2e12, 0.12, 0.7, 1), (13, 5e12, 0.09, 1.6, 0), (9, 1e12, 0.2, 1.1, 1),
                                 (20, 3e12, 0.1, 0.9, 0)):
         insts.append(small_cfg(n=side, bc=bc, r_init=r, exp_energy=E, dens_ambient=rho))
     ens = mc.Ensemble(insts, slab=6)
     assert ens.slab == 6
+    ens.set_stage_variant(flags)
+    assert ens.stage_variant == flags
     ens.init_data()
     ens.run(2, use_graph=False)
     ens.run(2, use_graph=True)

@@ -460,7 +460,9 @@ def main():
         with open(a.tiling_json) as f:
             prev = json.loads([ln for ln in f if ln.startswith("{")][-1])
         for st, b in prev["config"]["tiling"].items():
-            L.set_tiling(L.STAGES.index(st), *b)
+            L.set_tiling(L.STAGES.index(st), *b[:3])
+            if len(b) > 3:
+                L.set_brick_order(L.STAGES.index(st), b[3])
         L.set_stage_variant(prev["config"].get("stage_variant", 0))
         tuned = None
     else:
@@ -469,7 +471,8 @@ def main():
     if a.stage_variant >= 0:
         L.set_stage_variant(a.stage_variant)
     variant = L.stage_variant
-    tiling = {L.STAGES[st]: list(L.get_tiling(st)) for st in range(1, len(L.STAGES) - 1)}
+    # brick (bx, by, bz) and brick order of every stage
+    tiling = {L.STAGES[st]: list(L.get_tiling(st)) + [L.brick_order(st)] for st in range(1, len(L.STAGES) - 1)}
     L.run(a.warmup, use_graph=True)
     L.sync()
     torch.cuda.synchronize()

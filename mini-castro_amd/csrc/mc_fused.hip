@@ -1089,9 +1089,9 @@ static Plan plan_for(FusedLevel *F, const KernelTiling &t, int sh, int add, int 
 {
     const FInst *tab = slab < 0 ? F->inst : F->sinst[slab];
     for (auto &pb : F->plans)
-        if (pb.t.bx == t.bx && pb.t.by == t.by && pb.t.bz == t.bz && pb.shift == sh && pb.add == add &&
-            pb.slab == slab && pb.ovl == ovl)
-            return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+        if (pb.t.bx == t.bx && pb.t.by == t.by && pb.t.bz == t.bz && pb.t.zg == t.zg && pb.shift == sh &&
+            pb.add == add && pb.slab == slab && pb.ovl == ovl)
+            return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add, t.zg};
     FusedLevel::PlanBuf pb;
     pb.t = t;
     pb.shift = sh;
@@ -1119,9 +1119,9 @@ static Plan plan_for(FusedLevel *F, const KernelTiling &t, int sh, int add, int 
     pb.nblocks = acc;
     if (check_hip(hipMalloc(&pb.blk0, sizeof(int) * (F->ninst + 1)), "hipMalloc(plan)") ||
         check_hip(hipMemcpy(pb.blk0, b0.data(), sizeof(int) * (F->ninst + 1), hipMemcpyHostToDevice), "plan h2d"))
-        return Plan{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
+        return Plan{nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0};
     F->plans.push_back(pb);
-    return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add};
+    return Plan{tab, pb.blk0, F->ninst, t.bx, t.by, t.bz, sh, add, t.zg};
 }
 
 Plan get_plan(FusedLevel *F, int stage, int slab)
@@ -1744,7 +1744,7 @@ bool fused_tiling_ok(int stage, int bx, int by, int bz)
 void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz)
 {
     if (!fused_tiling_ok(stage, bx, by, bz)) return;
-    F->tiling[stage] = KernelTiling{bx, by, bz};
+    F->tiling[stage] = KernelTiling{bx, by, bz, F->tiling[stage].zg};
     if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
     if (F->graph) (void)hipGraphDestroy(F->graph);
     F->gexec = nullptr;
@@ -1757,6 +1757,20 @@ void fused_get_tiling(FusedLevel *F, int stage, int *bxyz)
     bxyz[1] = F->tiling[stage].by;
     bxyz[2] = F->tiling[stage].bz;
 }
+
+bool fused_brick_order_ok(int zg) { return zg == 0 || (zg >= 2 && zg <= 64); }
+
+void fused_set_brick_order(FusedLevel *F, int stage, int zg)
+{
+    if (stage < 0 || stage >= NSTAGE || !fused_brick_order_ok(zg)) return;
+    F->tiling[stage].zg = overlap_stage(stage) ? 0 : zg;
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    F->gexec = nullptr;
+    F->graph = nullptr;
+}
+
+int fused_brick_order(const FusedLevel *F, int stage) { return F->tiling[stage].zg; }
 
 int fused_run_stage(FusedLevel *F, int stage, hipStream_t s)
 {
@@ -1856,7 +1870,9 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
         double tbest = 1e300;
         std::vector<KernelTiling> cands(std::begin(tune_cand), std::end(tune_cand));
         if (overlap_stage(st)) cands.assign(std::begin(tflux_cand), std::end(tflux_cand));
-        for (const KernelTiling &c : cands) {
+        // reps launches of stage st with tiling c; the time per launch, or a
+        // negative value on an error (rc set)
+        auto time_tiling = [&](const KernelTiling &c) -> double {
             F->tiling[st] = c;
             float ms = 0.f;
             if (!writes_state) {
@@ -1866,7 +1882,7 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
                 (void)hipEventRecord(e1, s);
                 if ((rc = check_hip(hipEventSynchronize(e1), "autotune sync") ||
                           check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")))
-                    break;
+                    return -1.0;
             } else {
                 (void)get_plan(F, st);  // the block table, outside the timed region
                 for (int r = 0; r < reps && !rc; ++r) {
@@ -1880,13 +1896,30 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
                          check_hip(hipEventElapsedTime(&one, e0, e1), "hipEventElapsedTime");
                     ms += one;
                 }
-                if (rc) break;
+                if (rc) return -1.0;
             }
-            if (ms / reps < tbest) {
-                tbest = ms / reps;
+            return ms / reps;
+        };
+        for (const KernelTiling &c : cands) {
+            const double t = time_tiling(c);
+            if (t < 0) break;
+            if (t < tbest) {
+                tbest = t;
                 best = c;
             }
         }
+        // then the brick order of the best brick: groups of brick planes in z
+        if (!overlap_stage(st))
+            for (int zg : {2, 4, 8}) {
+                if (rc) break;
+                KernelTiling c = best;
+                c.zg = zg;
+                const double t = time_tiling(c);
+                if (t >= 0 && t < tbest) {
+                    tbest = t;
+                    best = c;
+                }
+            }
         F->tiling[st] = best;
         tb[st] = tbest;
         if (bricks) {

@@ -57,6 +57,7 @@ struct Plan {
     int bx, by, bz;
     int shift;        // first index: -shift
     int add;          // extent: n + add (cells [0,n): 0,0; faces [0,n]: 0,1; extended [-1,n]: 1,2)
+    int zg;           // brick order: 0 raster, else groups of zg brick planes (plan_cell_at)
 };
 
 struct Ctx {
@@ -87,7 +88,17 @@ __device__ __forceinline__ bool plan_cell_at(const Plan &p, int lb, int &m, int 
     const int e0 = I.n[0] + p.add, e1 = I.n[1] + p.add, e2 = I.zn + p.add;
     const int nb0 = (e0 + p.bx - 1) / p.bx, nb1 = (e1 + p.by - 1) / p.by;
     const int b = lb - p.blk0[m];
-    const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
+    int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
+    if (p.zg > 1) {
+        // groups of zg brick planes: x fastest, then z within the group, then
+        // y, so that a brick's z-neighbours run soon after it (the last
+        // group may hold fewer planes)
+        const int nb2 = (e2 + p.bz - 1) / p.bz;
+        const int r = b / nb0, grp = r / (nb1 * p.zg), rr = r - grp * nb1 * p.zg;
+        const int gz = min(p.zg, nb2 - grp * p.zg);
+        bk = grp * p.zg + rr % gz;
+        bj = rr / gz;
+    }
     const int t = threadIdx.x;
     const int ti = t % p.bx, tj = (t / p.bx) % p.by, tk = t / (p.bx * p.by);
     const int li = bi * p.bx + ti, lj = bj * p.by + tj, lk = bk * p.bz + tk;
@@ -380,6 +391,7 @@ struct ArenaBox {
 // ------------------------------------------------------------------ host side
 struct KernelTiling {
     int bx, by, bz;
+    int zg = 0;  // brick order (Plan::zg); bricks that overlap keep raster order
 };
 
 struct FusedLevel {

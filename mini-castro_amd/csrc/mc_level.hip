@@ -449,6 +449,24 @@ int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
     return 0;
 }
 
+int mc_level_set_brick_order(mc_level_t *L, int stage, int zgroup)
+{
+    if (!level_engine(L)) return -1;
+    if (stage < 0 || stage >= FST_N || !fused_brick_order_ok(zgroup)) {
+        set_error("mc_level_set_brick_order: zgroup must be 0 (raster) or 2..64 brick planes");
+        return -1;
+    }
+    fused_set_brick_order(L->fused, stage, zgroup);
+    return 0;
+}
+
+int mc_level_brick_order(mc_level_t *L, int stage)
+{
+    if (!level_engine(L)) return -1;
+    if (stage < 0 || stage >= FST_N) return -1;
+    return fused_brick_order(L->fused, stage);
+}
+
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz)
 {
     if (!level_engine(L)) return -1;

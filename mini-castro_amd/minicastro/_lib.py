@@ -93,6 +93,8 @@ SIGS = [
     ("mc_level_fused_slab", _I, [_vp]),
     ("mc_level_set_tiling", _I, [_vp, _I, _I, _I, _I]),
     ("mc_level_get_tiling", _I, [_vp, _I, _ip]),
+    ("mc_level_set_brick_order", _I, [_vp, _I, _I]),
+    ("mc_level_brick_order", _I, [_vp, _I]),
     ("mc_level_run_stage", _I, [_vp, _I]),
     ("mc_level_autotune", _I, [_vp, _I, _ip, _dp]),
     ("mc_level_set_stage_variant", _I, [_vp, _I]),

@@ -149,6 +149,16 @@ class Level:
         check(self.L.mc_level_get_tiling(self.h, int(stage), a), "mc_level_get_tiling")
         return tuple(a)
 
+    def set_brick_order(self, stage, zgroup):
+        """0: raster brick order; 2..64: groups of that many brick planes in z."""
+        check(self.L.mc_level_set_brick_order(self.h, int(stage), int(zgroup)), "mc_level_set_brick_order")
+
+    def brick_order(self, stage):
+        v = self.L.mc_level_brick_order(self.h, int(stage))
+        if v < 0:
+            raise RuntimeError("mc_level_brick_order: %s" % self.L.mc_last_error().decode())
+        return v
+
     def run_stage(self, stage):
         check(self.L.mc_level_run_stage(self.h, int(stage)), "mc_level_run_stage")
 

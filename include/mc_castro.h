@@ -281,10 +281,10 @@ int mc_level_fused_slab(mc_level_t *L);
    (multiple of 64, <= 256), X dividing W*L. */
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
-/* Order in which a stage's bricks are launched (stages whose bricks do not
-   overlap): 0 raster (x fastest, then y, then z), or zgroup = 2..64 brick
-   planes per group in z (x fastest, then z within the group, then y), which
-   keeps a brick's z-neighbours close in time.  Never changes results.
+/* Order in which a stage's bricks are launched: 0 raster (x fastest, then
+   y, then z), or zgroup = 2..64 brick planes per group in z (x fastest,
+   then z within the group, then y), which keeps a brick's z-neighbours
+   close in time.  Never changes results.
    mc_level_brick_order returns it (-1 on error). */
 int mc_level_set_brick_order(mc_level_t *L, int stage, int zgroup);
 int mc_level_brick_order(mc_level_t *L, int stage);
@@ -294,8 +294,8 @@ int mc_level_brick_order(mc_level_t *L, int stage);
 int mc_level_run_stage(mc_level_t *L, int stage);
 /* Fused mode: time every candidate brick of every stage of every kernel
    variant (reps launches each) on the data of one coarse step and keep the
-   fastest, then the brick order (zgroup 0, 2, 4, 8) of the stages whose
-   bricks do not overlap, then select the faster kernel of each alternative
+   fastest, then the brick order (zgroup 0, 2, 4, 8) of each stage, then
+   select the faster kernel of each alternative
    (the stage variant below; mc_level_stage_variant reports it); the state
    and the clock are restored.  bricks (11 x 3 ints) and best_ms (11) receive every stage's
    tiling and the tuned stages' time per launch (0 for est, which is not

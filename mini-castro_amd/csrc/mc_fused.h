@@ -55,9 +55,8 @@ int fused_read_est(FusedLevel *F, int m, double *est, hipStream_t s);
 // 64..256 threads and X dividing W*L
 bool fused_tiling_ok(int stage, int bx, int by, int bz);
 void fused_set_tiling(FusedLevel *F, int stage, int bx, int by, int bz);  // ignores an invalid tiling
-// Brick order of a stage whose bricks do not overlap: 0 raster (x, then y,
-// then z), or groups of zg brick planes in z (x, then z within the group,
-// then y).  Overlapping stages ignore it.
+// Brick order of a stage: 0 raster (x, then y, then z), or groups of zg
+// brick planes in z (x, then z within the group, then y).
 bool fused_brick_order_ok(int zg);
 void fused_set_brick_order(FusedLevel *F, int stage, int zg);
 int fused_brick_order(const FusedLevel *F, int stage);

@@ -1070,9 +1070,9 @@ static bool whole_instance_stage(int stage) { return stage == ST_EST || stage ==
 // brick of an overlapping stage's direction d for its tiling (W, L, X)
 static KernelTiling ovl_brick(const KernelTiling &t, int d)
 {
-    if (d == 0) return KernelTiling{t.bz, t.bx * t.by / t.bz, 1};
-    if (d == 1) return KernelTiling{t.bx, t.by, 1};
-    return KernelTiling{t.bx, 1, t.by};
+    if (d == 0) return KernelTiling{t.bz, t.bx * t.by / t.bz, 1, t.zg};
+    if (d == 1) return KernelTiling{t.bx, t.by, 1, t.zg};
+    return KernelTiling{t.bx, 1, t.by, t.zg};
 }
 
 // an overlapping stage's tiling is valid when every direction's brick spans
@@ -1763,7 +1763,7 @@ bool fused_brick_order_ok(int zg) { return zg == 0 || (zg >= 2 && zg <= 64); }
 void fused_set_brick_order(FusedLevel *F, int stage, int zg)
 {
     if (stage < 0 || stage >= NSTAGE || !fused_brick_order_ok(zg)) return;
-    F->tiling[stage].zg = overlap_stage(stage) ? 0 : zg;
+    F->tiling[stage].zg = zg;
     if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
     if (F->graph) (void)hipGraphDestroy(F->graph);
     F->gexec = nullptr;
@@ -1909,17 +1909,16 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
             }
         }
         // then the brick order of the best brick: groups of brick planes in z
-        if (!overlap_stage(st))
-            for (int zg : {2, 4, 8}) {
-                if (rc) break;
-                KernelTiling c = best;
-                c.zg = zg;
-                const double t = time_tiling(c);
-                if (t >= 0 && t < tbest) {
-                    tbest = t;
-                    best = c;
-                }
+        for (int zg : {2, 4, 8}) {
+            if (rc) break;
+            KernelTiling c = best;
+            c.zg = zg;
+            const double t = time_tiling(c);
+            if (t >= 0 && t < tbest) {
+                tbest = t;
+                best = c;
             }
+        }
         F->tiling[st] = best;
         tb[st] = tbest;
         if (bricks) {

@@ -135,7 +135,7 @@ __device__ __forceinline__ bool plan_cell_sub(const Plan &p, int ny, int &sub, i
 // first brick.  Every cell of the range is a layer l >= 1 of exactly one
 // brick except that ring cell.  The brick count along d is
 // ceil((e - 1) / (b - 1)) for an extent of e cells (b >= 2).  Bricks are
-// numbered x fastest, then y, then z.
+// numbered x fastest, then y, then z, or in groups of zg brick planes.
 __device__ __forceinline__ bool plan_cell_ovl(const Plan &p, int d, int lb, int &m, int &i, int &j, int &k, int &l)
 {
     m = find_inst(p, lb);
@@ -144,7 +144,14 @@ __device__ __forceinline__ bool plan_cell_ovl(const Plan &p, int d, int lb, int 
     const int s0 = p.bx - (d == 0), s1 = p.by - (d == 1), s2 = p.bz - (d == 2);
     const int nb0 = (e0 - (d == 0) + s0 - 1) / s0, nb1 = (e1 - (d == 1) + s1 - 1) / s1;
     const int b = lb - p.blk0[m];
-    const int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
+    int bi = b % nb0, bj = (b / nb0) % nb1, bk = b / (nb0 * nb1);
+    if (p.zg > 1) {  // groups of zg brick planes, as in plan_cell_at
+        const int nb2 = (e2 - (d == 2) + s2 - 1) / s2;
+        const int r = b / nb0, grp = r / (nb1 * p.zg), rr = r - grp * nb1 * p.zg;
+        const int gz = min(p.zg, nb2 - grp * p.zg);
+        bk = grp * p.zg + rr % gz;
+        bj = rr / gz;
+    }
     const int t = threadIdx.x;
     const int ti = t % p.bx, tj = (t / p.bx) % p.by, tk = t / (p.bx * p.by);
     const int li = bi * s0 + ti, lj = bj * s1 + tj, lk = bk * s2 + tk;
@@ -391,7 +398,7 @@ struct ArenaBox {
 // ------------------------------------------------------------------ host side
 struct KernelTiling {
     int bx, by, bz;
-    int zg = 0;  // brick order (Plan::zg); bricks that overlap keep raster order
+    int zg = 0;  // brick order (Plan::zg)
 };
 
 struct FusedLevel {

@@ -80,8 +80,7 @@ def test_tiling_does_not_change_results():
                   (1, (256, 1, 1)), (7, (8, 8, 2)), (8, (8, 16, 32)), (9, (32, 4, 128)), (10, (16, 4, 32))):
         B.set_tiling(st, *t)
         C.set_tiling(st, *t)
-        if st < 8:
-            B.set_brick_order(st, (2, 3, 4, 8)[st % 4])
+        B.set_brick_order(st, (2, 3, 4, 8)[st % 4])
     assert B.brick_order(6) == 4 and C.brick_order(6) == 0
     A.run(3, use_graph=False)
     B.run(3, use_graph=True)

@@ -34,7 +34,8 @@ def test_overlap_bricks_match_oracle(t):
     """20^3 in uneven boxes (22 ring cells per direction: one brick of L = 32,
     or 7, 11 or 21 overlapping bricks), outflow and periodic, two eager steps
     and a graph replay; the three overlapping stages take bricks of different
-    shapes (the final stage by cells on every other tiling)."""
+    shapes and brick orders (the final stage by cells on every other
+    tiling)."""
     bc = TILINGS.index(t) % 2
     cfg = small_cfg(n=20, bc=bc, r_init=0.13)
     boxes = configs.uneven_boxes(20, seed=7, parts=2, sizes=(4, 8, 12))
@@ -45,6 +46,8 @@ def test_overlap_bricks_match_oracle(t):
     G.set_tiling(G.STAGES.index("finalc"), *TILINGS[(TILINGS.index(t) + 6) % len(TILINGS)])
     if TILINGS.index(t) % 2:
         G.set_stage_variant(G.CELL_FINAL)
+    for st in ("tflux", "trans1c", "finalc"):  # raster, or groups of brick planes in z
+        G.set_brick_order(G.STAGES.index(st), (0, 2, 3, 4, 8)[(TILINGS.index(t) + len(st)) % 5])
     G.init_data(**cfg)
     for s in range(2):
         dto, dtg = O.coarse_step(), G.coarse_step()

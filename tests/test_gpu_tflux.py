@@ -110,3 +110,28 @@ def test_bad_variant_refused():
     with pytest.raises(RuntimeError):
         L.set_tiling(L.STAGES.index("trans1c"), 16, 16, 3)
     L.close()
+
+
+@pytest.mark.parametrize("flags,slab", [(0, 0), (4, 3), (3, 0)])
+def test_tiny_instances_match_sequential_runs(flags, slab):
+    """An ensemble of instances smaller than every brick (sides 4, 5, 7; one
+    brick spans a whole direction, most threads of a brick are idle), with
+    and without z-slabs and with each kernel set: every instance equals its
+    per-box run over three steps."""
+    insts = []
+    for side, bc, r in ((4, 1, 0.3), (5, 0, 0.25), (7, 1, 0.2), (5, 1, 0.35)):
+        insts.append(small_cfg(n=side, bc=bc, r_init=r))
+    ens = mc.Ensemble(insts, slab=slab)
+    ens.set_stage_variant(flags)
+    ens.init_data()
+    ens.run(3, use_graph=True)
+    for m, d in enumerate(insts):
+        L = mc.Level(d["n_cell"], d["boxes"], d["prob_lo"], d["prob_hi"], d["bc"], mc.MODE_PLAIN)
+        L.init_data(**d)
+        for _ in range(3):
+            L.coarse_step()
+        dt, t, steps = ens.clock(m)
+        assert steps == 3 and dt == L.dt and t == L.time, (m, dt, L.dt)
+        np.testing.assert_array_equal(ens.state(m), L.state(), err_msg="instance %d" % m)
+        L.close()
+    ens.close()

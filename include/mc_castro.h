@@ -353,15 +353,19 @@ int mc_flux_register_reflux(mc_flux_register_t *R, double *state, const int *sta
    cells c of box (lo, hi), ncomp components */
 int mc_average_down(const double *fine, const int *fine_lo, const int *fine_hi, double *crse, const int *crse_lo,
                     const int *crse_hi, const int *lo, const int *hi, const int *ratio, int ncomp);
-/* Per-box path levels: crse_side is the register of the next finer level,
-   which this level's steps initialise with their store_flux output (mult
-   -1); fine_side is this level's own register, to which the steps of its
-   boxes that are fine boxes of the register add their fluxes (mult +1).
-   NULL detaches. */
+/* crse_side is the register of the next finer level, which this level's
+   steps initialise with their store_flux output (mult -1); fine_side is this
+   level's own register, to which the steps of its boxes that are fine boxes
+   of the register add their fluxes (mult +1).  NULL detaches.  A fused-mode
+   level can be the fine side (its final fluxes feed FineAdd; whole-level
+   face arrays, no z-slabs), not the coarse side: the fused step folds
+   post_timestep's clean_state in, and a coarse level is refluxed and
+   averaged down before that clean. */
 int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc_flux_register_t *fine_side);
 /* Castro::reflux into this coarse level's state, then the register is zeroed */
 int mc_level_reflux(mc_level_t *L, mc_flux_register_t *R);
-/* Castro::avgDown from the given boxes of the fine level (all if nfine < 0) */
+/* Castro::avgDown from the given boxes of the fine level (all if nfine < 0);
+   either level may be in fused mode */
 int mc_level_average_down(mc_level_t *fine, mc_level_t *crse, const int *ratio, int nfine, const int *fine_box_ids);
 
 /* ---- batched advance over caller-owned FABs ----

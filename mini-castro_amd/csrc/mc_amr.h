@@ -47,6 +47,12 @@ int flux_register_crse_init(mc_flux_register *R, const double *flux, const Box &
                             hipStream_t s);
 int flux_register_fine_add(mc_flux_register *R, int fine_box, const double *flux, const Box &fbox, int dir,
                            double mult, hipStream_t s);
+// FineAdd of every fine box of the register from the fused engine of the
+// fine level (its final fluxes of the step just enqueued): dt and the face
+// areas of the three directions as store_flux takes them
+struct FusedLevel;
+int flux_register_fine_add_fused(mc_flux_register *R, const FusedLevel *F, double dt, const double *area,
+                                 double mult, hipStream_t s);
 int flux_register_clear_internal(mc_flux_register *R, hipStream_t s);
 int flux_register_fill(mc_flux_register *R, double v, hipStream_t s);
 int flux_register_reflux(mc_flux_register *R, double *state, const Box &sbox, const Box &valid, const double *vol,

@@ -32,6 +32,8 @@
 
 #include "mc_amr.h"
 #include "mc_common.h"
+#include "mc_eos.h"
+#include "mc_fused_impl.h"
 #include "mc_internal.h"
 #include "../../include/mc_castro.h"
 
@@ -80,6 +82,48 @@ __global__ void k_reg_fine_add(Box fb, long n, double *reg, Arr flx, int dir, in
             const int kk = r2 * k;
             for (int joff = 0; joff < r1; ++joff)
                 for (int ioff = 0; ioff < r0; ++ioff) r += mult * flx(r0 * i + ioff, r1 * j + joff, kk, c);
+        }
+        reg[l + n * c] = r;
+    }
+}
+
+// FineAdd from the fused engine's final fluxes (one instance covering the
+// fine level, whole-instance face arrays): component c of face (i,j,k) in
+// direction dir is FN[dir] at eidx, and store_flux's dt * F * area is formed
+// as k_store_flux forms it before the sum of k_reg_fine_add
+__device__ __forceinline__ double fn_component(double *FN, const FInst &I, int c, idx_t e)
+{
+    if (c < UFS) return EF(FN, I, NFN, c, e);
+    const int sp = c - UFS;
+    const dbl2 v = SP2(species_pairs(FN, I, NFN, FN_X), sp >> 1, e);
+    return (sp & 1) ? v.y : v.x;
+}
+
+__global__ void k_reg_fine_add_fn(Box fb, long n, double *reg, double *FN, const FInst *inst, int dir, int r0,
+                                  int r1, int r2, double dt, double area, double mult)
+{
+    int i, j, k;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!iter_cell(make_iter(fb), t, i, j, k)) return;
+    const FInst &I = inst[0];
+    const long l = face_local(fb, i, j, k);
+    for (int c = 0; c < NVAR; ++c) {
+        double r = reg[l + n * c];
+        if (dir == 0) {
+            const int ii = r0 * i;
+            for (int koff = 0; koff < r2; ++koff)
+                for (int joff = 0; joff < r1; ++joff)
+                    r += mult * (dt * fn_component(FN, I, c, eidx(I, ii, r1 * j + joff, r2 * k + koff)) * area);
+        } else if (dir == 1) {
+            const int jj = r1 * j;
+            for (int koff = 0; koff < r2; ++koff)
+                for (int ioff = 0; ioff < r0; ++ioff)
+                    r += mult * (dt * fn_component(FN, I, c, eidx(I, r0 * i + ioff, jj, r2 * k + koff)) * area);
+        } else {
+            const int kk = r2 * k;
+            for (int joff = 0; joff < r1; ++joff)
+                for (int ioff = 0; ioff < r0; ++ioff)
+                    r += mult * (dt * fn_component(FN, I, c, eidx(I, r0 * i + ioff, r1 * j + joff, kk)) * area);
         }
         reg[l + n * c] = r;
     }
@@ -212,6 +256,28 @@ int flux_register_fine_add(mc_flux_register *R, int fine_box, const double *flux
                            dir, R->ratio[0], R->ratio[1], R->ratio[2], mult);
     }
     if (check_hip(hipGetLastError(), "flux register FineAdd")) return -1;
+    return reg_end(R, s);
+}
+
+int flux_register_fine_add_fused(mc_flux_register *R, const FusedLevel *F, double dt, const double *area,
+                                 double mult, hipStream_t s)
+{
+    if (F->ninst != 1 || F->slab) {
+        set_error("flux register FineAdd: the fused level must hold one instance with whole face arrays "
+                  "(no z-slabs)");
+        return -1;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (F->hinst[0].n[d] != R->crse_n[d] * R->ratio[d]) {
+            set_error("flux register FineAdd: the fused level is not the register's refined domain");
+            return -1;
+        }
+    if (reg_begin(R, s)) return -1;
+    for (const RegFace &f : R->faces)
+        hipLaunchKernelGGL(k_reg_fine_add_fn, dim3(grid_for(f.n)), dim3(kBlock), 0, s, f.fb, f.n, R->reg + f.off,
+                           F->ctx.FN[f.dir], F->inst, f.dir, R->ratio[0], R->ratio[1], R->ratio[2], dt,
+                           area[f.dir], mult);
+    if (check_hip(hipGetLastError(), "flux register FineAdd (fused)")) return -1;
     return reg_end(R, s);
 }
 

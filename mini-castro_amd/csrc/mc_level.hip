@@ -728,13 +728,19 @@ int mc_level_advance(mc_level_t *L, double dt)
         return -1;
     }
     if (L->mode == MC_MODE_FUSED) {
-        if (L->reg_crse || L->reg_fine) {
-            set_error("mc_level_advance: flux registers are filled by the per-box path (MC_MODE_PLAIN)");
+        if (L->reg_crse) {
+            set_error("mc_level_advance: a fused level cannot be the coarse side of a flux register");
             return -1;
         }
         // the fused update includes post_timestep's clean_state
         if (!L->fused_ahead && sync_to_fused(L)) return -1;
-        return fused_advance_dt(L->fused, dt, L->stream);
+        if (fused_advance_dt(L->fused, dt, L->stream)) return -1;
+        if (!L->reg_fine) return 0;
+        // FineAdd (Source/Castro_advance.cpp:86-88) from the step's final
+        // fluxes, with store_flux's face areas (Geometry::GetFaceArea)
+        const double *dx = L->dx;
+        const double ar[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
+        return flux_register_fine_add_fused(L->reg_fine, L->fused, dt, ar, 1.0, L->stream);
     }
     std::swap(L->S_old, L->S_new); /* swapTimeLevels, Castro_advance.cpp:32-33 */
     if (alloc_plain(L)) return -1;
@@ -868,8 +874,13 @@ double mc_level_dt(const mc_level_t *L) { return L->dt; }
 // ---- AMR coupling between two levels (per-box path) ----
 int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc_flux_register_t *fine_side)
 {
-    if ((crse_side || fine_side) && L->mode != MC_MODE_PLAIN) {
-        set_error("mc_level_set_flux_registers: flux registers are filled by the per-box path (MC_MODE_PLAIN)");
+    // A fused level folds post_timestep's clean_state into its step, while a
+    // coarse level must be refluxed and averaged down before that clean
+    // (Castro::post_timestep, Source/Castro.cpp:343-369): only the per-box
+    // path can be the coarse side.  A fused level can be the fine side.
+    if (crse_side && L->mode != MC_MODE_PLAIN) {
+        set_error("mc_level_set_flux_registers: the coarse side of a register runs the per-box path "
+                  "(MC_MODE_PLAIN); a fused level can only be the fine side");
         return -1;
     }
     for (int d = 0; d < 3; ++d) {
@@ -931,10 +942,9 @@ int mc_level_reflux(mc_level_t *L, mc_flux_register_t *R)
 // fine cells
 int mc_level_average_down(mc_level_t *fine, mc_level_t *crse, const int *ratio, int nfine, const int *fine_box_ids)
 {
-    if (fine->mode != MC_MODE_PLAIN || crse->mode != MC_MODE_PLAIN) {
-        set_error("mc_level_average_down: per-box path (MC_MODE_PLAIN) only");
-        return -1;
-    }
+    // a fused level hands its state to S_new first; a fused coarse level
+    // takes the averaged S_new back at its next step
+    if (sync_from_fused(fine) || sync_from_fused(crse)) return -1;
     for (int d = 0; d < 3; ++d)
         if (ratio[d] < 1 || crse->n_cell[d] * ratio[d] != fine->n_cell[d]) {
             set_error("mc_level_average_down: the levels' domains do not differ by the ratio");

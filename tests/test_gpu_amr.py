@@ -145,3 +145,46 @@ def test_two_level_sedov_matches_oracle_and_conserves():
     F = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mc.MODE_FUSED)
     with pytest.raises(RuntimeError):
         amr.set_flux_registers(F, crse_side=Rg)
+
+
+@pytest.mark.parametrize("fine_slab", [0, 5])
+def test_two_level_sedov_with_a_fused_fine_level(fine_slab):
+    """The same two-level step with the fine level on the fused engine: its
+    FineAdd comes from the engine's final fluxes, its average-down from the
+    engine's state, and both levels still equal the oracle bit for bit.  A
+    fused level cannot be a register's coarse side, nor feed one from
+    z-slabs."""
+    from test_oracle_amr import two_level
+    nc = 16
+    Lc, Lf, R, region, totals = two_level(nc=nc, nthreads=8)
+    cfg_c = small_cfg(n=nc, bc=1, r_init=0.15)
+    cfg_c["boxes"] = configs.grid_boxes(nc, 8)
+    cfg_f = small_cfg(n=2 * nc, bc=1, r_init=0.15)
+    cfg_f["boxes"] = configs.grid_boxes(2 * nc, 8)
+    Gc = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mc.MODE_PLAIN)
+    Gf = mc.Level(cfg_f["n_cell"], cfg_f["boxes"], cfg_f["prob_lo"], cfg_f["prob_hi"], 1, mc.MODE_FUSED)
+    Gf.set_fused_slab(fine_slab)
+    Gc.init_data(**cfg_c)
+    Gf.init_data(**cfg_f)
+    Rg = amr.FluxRegister(cfg_f["boxes"][region], RATIO, (nc,) * 3, 1)
+    amr.set_flux_registers(Gc, crse_side=Rg)
+    amr.set_flux_registers(Gf, fine_side=Rg)
+    amr.average_down_levels(Gf, Gc, RATIO, region)
+    dt = Gc.initial_dt()
+    Gc.advance(dt)
+    if fine_slab:
+        with pytest.raises(RuntimeError):
+            Gf.advance(dt / 2)
+        return
+    for _ in range(2):
+        Gf.advance(dt / 2)
+        Gf.post_timestep()
+    amr.reflux(Gc, Rg)
+    amr.average_down_levels(Gf, Gc, RATIO, region)
+    Gc.post_timestep()
+    np.testing.assert_array_equal(Gf.state(), Lf.state())
+    np.testing.assert_array_equal(Gc.state(), Lc.state())
+    assert (Rg.data() == 0).all()
+    F = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mc.MODE_FUSED)
+    with pytest.raises(RuntimeError):
+        amr.set_flux_registers(F, crse_side=Rg)

@@ -357,10 +357,12 @@ int mc_average_down(const double *fine, const int *fine_lo, const int *fine_hi, 
    steps initialise with their store_flux output (mult -1); fine_side is this
    level's own register, to which the steps of its boxes that are fine boxes
    of the register add their fluxes (mult +1).  NULL detaches.  A fused-mode
-   level can be the fine side (its final fluxes feed FineAdd; whole-level
-   face arrays, no z-slabs), not the coarse side: the fused step folds
-   post_timestep's clean_state in, and a coarse level is refluxed and
-   averaged down before that clean. */
+   level feeds the register from its final fluxes (whole-level face arrays,
+   no z-slabs); as a coarse side its step stops after the advance's
+   clean_state, and mc_level_post_timestep applies post_timestep's clean
+   after the reflux and average-down, as Castro::post_timestep does.  A
+   level with registers steps through mc_level_advance and
+   mc_level_post_timestep (mc_level_run refuses). */
 int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc_flux_register_t *fine_side);
 /* Castro::reflux into this coarse level's state, then the register is zeroed */
 int mc_level_reflux(mc_level_t *L, mc_flux_register_t *R);

@@ -53,10 +53,14 @@ int flux_register_fine_add(mc_flux_register *R, int fine_box, const double *flux
 struct FusedLevel;
 int flux_register_fine_add_fused(mc_flux_register *R, const FusedLevel *F, double dt, const double *area,
                                  double mult, hipStream_t s);
+// CrseInit of every register face from the fused engine of the coarse level
+int flux_register_crse_init_fused(mc_flux_register *R, const FusedLevel *F, double dt, const double *area,
+                                  double mult, hipStream_t s);
 int flux_register_clear_internal(mc_flux_register *R, hipStream_t s);
 int flux_register_fill(mc_flux_register *R, double v, hipStream_t s);
+// vol null: every cell's volume is vconst
 int flux_register_reflux(mc_flux_register *R, double *state, const Box &sbox, const Box &valid, const double *vol,
-                         const Box &vbox, double scale, hipStream_t s);
+                         const Box &vbox, double scale, hipStream_t s, double vconst = 0.0);
 int average_down(const double *fine, const Box &fbox, double *crse, const Box &cbox, const Box &region,
                  const int *ratio, int ncomp, hipStream_t s);
 

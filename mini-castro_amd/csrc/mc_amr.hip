@@ -139,8 +139,8 @@ __global__ void k_reg_clear(long n, double *reg, const unsigned char *keep)
 
 // Reflux of one register face into the coarse cell just outside it, if that
 // cell (after the periodic wrap) lies in `valid`
-__global__ void k_reg_reflux(Box fb, long n, const double *reg, int dir, int side, Arr s, Arr vol, Box valid, int n0,
-                             int n1, int n2, int periodic, double scale)
+__global__ void k_reg_reflux(Box fb, long n, const double *reg, int dir, int side, Arr s, Arr vol, double vconst,
+                             Box valid, int n0, int n1, int n2, int periodic, double scale)
 {
     int i, j, k;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,10 +155,13 @@ __global__ void k_reg_reflux(Box fb, long n, const double *reg, int dir, int sid
     }
     for (int d = 0; d < 3; ++d)
         if (c[d] < valid.lo[d] || c[d] > valid.hi[d]) return;
+    // the cell volume: the volume FAB, or one value for every cell (vol
+    // without data)
+    const double v = vol.p ? vol(c[0], c[1], c[2], 0) : vconst;
     for (int q = 0; q < NVAR; ++q) {
         const double f = reg[l + n * q];
-        if (side == 0) s(c[0], c[1], c[2], q) += -scale * f / vol(c[0], c[1], c[2], 0);
-        else s(c[0], c[1], c[2], q) += scale * f / vol(c[0], c[1], c[2], 0);
+        if (side == 0) s(c[0], c[1], c[2], q) += -scale * f / v;
+        else s(c[0], c[1], c[2], q) += scale * f / v;
     }
 }
 
@@ -281,6 +284,42 @@ int flux_register_fine_add_fused(mc_flux_register *R, const FusedLevel *F, doubl
     return reg_end(R, s);
 }
 
+// CrseInit of every register face from the fused engine of the coarse
+// level: register = mult * (dt * F * area), as store_flux and
+// k_reg_crse_init form it
+__global__ void k_reg_crse_init_fn(Box fb, long n, double *reg, double *FN, const FInst *inst, double dt,
+                                   double area, double mult)
+{
+    int i, j, k;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!iter_cell(make_iter(fb), t, i, j, k)) return;
+    const FInst &I = inst[0];
+    const idx_t e = eidx(I, i, j, k);
+    const long l = face_local(fb, i, j, k);
+    for (int c = 0; c < NVAR; ++c) reg[l + n * c] = (dt * fn_component(FN, I, c, e) * area) * mult;
+}
+
+int flux_register_crse_init_fused(mc_flux_register *R, const FusedLevel *F, double dt, const double *area,
+                                  double mult, hipStream_t s)
+{
+    if (F->ninst != 1 || F->slab) {
+        set_error("flux register CrseInit: the fused level must hold one instance with whole face arrays "
+                  "(no z-slabs)");
+        return -1;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (F->hinst[0].n[d] != R->crse_n[d]) {
+            set_error("flux register CrseInit: the fused level is not the register's coarse domain");
+            return -1;
+        }
+    if (reg_begin(R, s)) return -1;
+    for (const RegFace &f : R->faces)
+        hipLaunchKernelGGL(k_reg_crse_init_fn, dim3(grid_for(f.n)), dim3(kBlock), 0, s, f.fb, f.n, R->reg + f.off,
+                           F->ctx.FN[f.dir], F->inst, dt, area[f.dir], mult);
+    if (check_hip(hipGetLastError(), "flux register CrseInit (fused)")) return -1;
+    return reg_end(R, s);
+}
+
 int flux_register_clear_internal(mc_flux_register *R, hipStream_t s)
 {
     long koff = 0;
@@ -294,13 +333,13 @@ int flux_register_clear_internal(mc_flux_register *R, hipStream_t s)
 }
 
 int flux_register_reflux(mc_flux_register *R, double *state, const Box &sbox, const Box &valid, const double *vol,
-                         const Box &vbox, double scale, hipStream_t s)
+                         const Box &vbox, double scale, hipStream_t s, double vconst)
 {
-    if (!inside(valid, sbox) || !inside(valid, vbox)) {
+    if (!inside(valid, sbox) || (vol && !inside(valid, vbox))) {
         set_error("flux register Reflux: the state or volume FAB does not hold the box");
         return -1;
     }
-    const Arr S = make_arr(state, sbox), V = make_arr(vol, vbox);
+    const Arr S = make_arr(state, sbox), V = make_arr(vol, vol ? vbox : sbox);
     if (reg_begin(R, s)) return -1;
     // orientations in AMReX's order: x, y, z low faces, then x, y, z high faces
     for (int side = 0; side < 2; ++side)
@@ -308,8 +347,8 @@ int flux_register_reflux(mc_flux_register *R, double *state, const Box &sbox, co
             for (const RegFace &f : R->faces) {
                 if (f.dir != dir || f.side != side) continue;
                 hipLaunchKernelGGL(k_reg_reflux, dim3(grid_for(f.n)), dim3(kBlock), 0, s, f.fb, f.n, R->reg + f.off,
-                                   dir, side, S, V, valid, R->crse_n[0], R->crse_n[1], R->crse_n[2], R->periodic,
-                                   scale);
+                                   dir, side, S, V, vconst, valid, R->crse_n[0], R->crse_n[1], R->crse_n[2],
+                                   R->periodic, scale);
             }
     if (check_hip(hipGetLastError(), "flux register Reflux")) return -1;
     return reg_end(R, s);
@@ -484,6 +523,10 @@ int mc_flux_register_reflux(mc_flux_register_t *R, double *state, const int *sta
                             const int *lo, const int *hi, const double *vol, const int *vol_lo, const int *vol_hi,
                             double scale)
 {
+    if (!vol) {
+        set_error("mc_flux_register_reflux: no volume FAB");
+        return -1;
+    }
     return flux_register_reflux(R, state, make_box(state_lo, state_hi), make_box(lo, hi), vol,
                                 make_box(vol_lo, vol_hi), scale, cur_stream());
 }

@@ -69,6 +69,9 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
 // K_finalc instead of K_final.  Results are the same bit for bit.
 enum { FV_SPLIT_TRACE = 1, FV_FACE_TRANS1 = 2, FV_CELL_FINAL = 4, FV_ALL = 7 };
 void fused_set_variant(FusedLevel *F, int flags);
+// true: the step ends after the advance's clean_state (no post_timestep clean,
+// no estdt), for a coarse level whose post_timestep follows a reflux
+void fused_set_defer_post(FusedLevel *F, bool defer);
 int fused_variant(const FusedLevel *F);
 
 // The arrays a step writes besides the state (primitives, flattening,

@@ -921,7 +921,12 @@ __global__ __launch_bounds__(256) void K_finalc(Plan3 p3, Ctx x)
 // Held to 256 registers (2 waves per SIMD): the few spilled values cost less
 // than the second wave gains on the EOS table gathers (6.1 -> 5.5 ms at 256^3;
 // the same limit slows K_prim, which spills more).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K_update(Plan p, Ctx x)
+// POST: post_timestep's clean_state and the estdt of the new state follow
+// the advance's clean (the step of a level that is not the coarse side of a
+// flux register); without it the state is left as Castro::advance leaves
+// it, for the reflux and average-down that come before that clean.
+template <bool POST>
+__device__ __forceinline__ void update_cells(Plan p, Ctx x)
 {
     int m, i, j, k;
     double v = DBL_MAX;
@@ -972,13 +977,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K
             u[n + 1] = u[n + 1] + dt * s1;
         }
         clean_cell(u, x.tab); /* Castro_advance.cpp:78 */
-        clean_cell(u, x.tab); /* post_timestep, Castro.cpp:369 */
+        if (POST) clean_cell(u, x.tab); /* post_timestep, Castro.cpp:369 */
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) CA(x.U, I, NVAR, n, c) = u[n];
-        v = estdt_cell(u, dx0, dx1, dx2, x.tab);
+        if (POST) v = estdt_cell(u, dx0, dx1, dx2, x.tab);
     }
-    v = block_min<256>(v);
-    if (threadIdx.x == 0) atomic_min_d(&x.dt_est[m], v);
+    if (POST) {
+        v = block_min<256>(v);
+        if (threadIdx.x == 0) atomic_min_d(&x.dt_est[m], v);
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K_update(Plan p, Ctx x)
+{
+    update_cells<true>(p, x);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void K_update_adv(Plan p, Ctx x)
+{
+    update_cells<false>(p, x);
 }
 
 // Time-step control of Castro::computeInitialDt / computeNewDt
@@ -1187,7 +1204,10 @@ static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1, in
     case ST_FLUX1: hipLaunchKernelGGL(K_flux1, g, b, 0, s, p, F->ctx); break;
     case ST_TRANS1: hipLaunchKernelGGL(K_trans1, g, b, 0, s, p, F->ctx); break;
     case ST_FINAL: hipLaunchKernelGGL(K_final, g, b, 0, s, p, F->ctx); break;
-    case ST_UPDATE: hipLaunchKernelGGL(K_update, g, b, 0, s, p, F->ctx); break;
+    case ST_UPDATE:
+        if (F->defer_post) hipLaunchKernelGGL(K_update_adv, g, b, 0, s, p, F->ctx);
+        else hipLaunchKernelGGL(K_update, g, b, 0, s, p, F->ctx);
+        break;
     }
 }
 
@@ -1733,6 +1753,15 @@ void fused_set_variant(FusedLevel *F, int flags)
 }
 
 int fused_variant(const FusedLevel *F) { return F->variant; }
+
+void fused_set_defer_post(FusedLevel *F, bool defer)
+{
+    F->defer_post = defer;
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    F->gexec = nullptr;
+    F->graph = nullptr;
+}
 
 bool fused_tiling_ok(int stage, int bx, int by, int bz)
 {

@@ -423,6 +423,9 @@ struct FusedLevel {
     // neighbour along that direction by one cell
     KernelTiling tiling[FST_N];
     int variant = 0;  // FV_* flags
+    // K_update without post_timestep's clean and estdt (a coarse level of a
+    // flux register, whose post_timestep runs after the reflux)
+    bool defer_post = false;
     // z-slabs of the face stages: planes per slab (0: the whole instance),
     // slab count, and one instance table per slab (zlo, zn set; slab 0 of a
     // whole-instance engine is `inst` itself)

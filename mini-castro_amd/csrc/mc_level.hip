@@ -168,6 +168,9 @@ struct mc_level {
     std::vector<int> fine_index;
     int fused_slab_req = -1;  // z-slab planes of the fused engine: -1 chosen by memory, 0 whole level
     int fused_variant = 0;  // kernel variant flags of the fused engine (FV_*)
+    // the fused step stops after the advance's clean_state; post_timestep
+    // runs separately (a coarse level of a flux register)
+    bool defer_post = false;
 };
 
 static double *dalloc(size_t n, bool &ok)
@@ -377,7 +380,10 @@ static FusedLevel *level_engine(mc_level_t *L)
         slab = fused_auto_slab({sp});
     }
     L->fused = fused_create(L->boxes, L->n_cell, L->bc, L->dx, slab);
-    if (L->fused) fused_set_variant(L->fused, L->fused_variant);
+    if (L->fused) {
+        fused_set_variant(L->fused, L->fused_variant);
+        fused_set_defer_post(L->fused, L->defer_post);
+    }
     return L->fused;
 }
 
@@ -728,19 +734,18 @@ int mc_level_advance(mc_level_t *L, double dt)
         return -1;
     }
     if (L->mode == MC_MODE_FUSED) {
-        if (L->reg_crse) {
-            set_error("mc_level_advance: a fused level cannot be the coarse side of a flux register");
-            return -1;
-        }
-        // the fused update includes post_timestep's clean_state
+        // the fused update includes post_timestep's clean_state unless the
+        // level is the coarse side of a flux register
         if (!L->fused_ahead && sync_to_fused(L)) return -1;
         if (fused_advance_dt(L->fused, dt, L->stream)) return -1;
-        if (!L->reg_fine) return 0;
-        // FineAdd (Source/Castro_advance.cpp:86-88) from the step's final
-        // fluxes, with store_flux's face areas (Geometry::GetFaceArea)
+        // CrseInit / FineAdd (Source/Castro_advance.cpp:82-88) from the
+        // step's final fluxes, with store_flux's face areas
+        // (Geometry::GetFaceArea)
         const double *dx = L->dx;
         const double ar[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
-        return flux_register_fine_add_fused(L->reg_fine, L->fused, dt, ar, 1.0, L->stream);
+        if (L->reg_crse && flux_register_crse_init_fused(L->reg_crse, L->fused, dt, ar, -1.0, L->stream)) return -1;
+        if (L->reg_fine && flux_register_fine_add_fused(L->reg_fine, L->fused, dt, ar, 1.0, L->stream)) return -1;
+        return 0;
     }
     std::swap(L->S_old, L->S_new); /* swapTimeLevels, Castro_advance.cpp:32-33 */
     if (alloc_plain(L)) return -1;
@@ -762,7 +767,10 @@ int mc_level_advance(mc_level_t *L, double dt)
 
 int mc_level_post_timestep(mc_level_t *L)
 {
-    if (L->mode == MC_MODE_FUSED) return 0; /* folded into the fused update */
+    if (L->mode == MC_MODE_FUSED && !L->defer_post) return 0; /* folded into the fused update */
+    // a fused coarse level of a register: the clean runs on S_new, which the
+    // engine takes back at the next step
+    if (L->mode == MC_MODE_FUSED && sync_from_fused(L)) return -1;
     void *prev = mc_get_stream();
     mc_set_stream(L->stream);
     clean_state(L, L->S_new, L->off_valid, 0); /* Castro.cpp:369 */
@@ -772,6 +780,11 @@ int mc_level_post_timestep(mc_level_t *L)
 
 int mc_level_run(mc_level_t *L, int nsteps, double stop_time, int use_graph)
 {
+    if (L->reg_crse || L->reg_fine) {
+        set_error("mc_level_run: a level with flux registers steps through mc_level_advance and "
+                  "mc_level_post_timestep (the AMR driver's order)");
+        return -1;
+    }
     if (L->mode == MC_MODE_FUSED) {
         if (!L->fused_ahead && sync_to_fused(L)) return -1;
         if (fused_steps(L->fused, nsteps, stop_time, L->stream, use_graph != 0)) return -1;
@@ -786,6 +799,10 @@ int mc_level_run_timed(mc_level_t *L, int nsteps, double stop_time, double *stag
 {
     if (L->mode != MC_MODE_FUSED) {
         set_error("mc_level_run_timed: fused mode only");
+        return -1;
+    }
+    if (L->reg_crse || L->reg_fine) {
+        set_error("mc_level_run_timed: not with flux registers attached");
         return -1;
     }
     if (!L->fused_ahead && sync_to_fused(L)) return -1;
@@ -874,15 +891,7 @@ double mc_level_dt(const mc_level_t *L) { return L->dt; }
 // ---- AMR coupling between two levels (per-box path) ----
 int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc_flux_register_t *fine_side)
 {
-    // A fused level folds post_timestep's clean_state into its step, while a
-    // coarse level must be refluxed and averaged down before that clean
-    // (Castro::post_timestep, Source/Castro.cpp:343-369): only the per-box
-    // path can be the coarse side.  A fused level can be the fine side.
-    if (crse_side && L->mode != MC_MODE_PLAIN) {
-        set_error("mc_level_set_flux_registers: the coarse side of a register runs the per-box path "
-                  "(MC_MODE_PLAIN); a fused level can only be the fine side");
-        return -1;
-    }
+
     for (int d = 0; d < 3; ++d) {
         if (crse_side && crse_side->crse_n[d] != L->n_cell[d]) {
             set_error("mc_level_set_flux_registers: the register's coarse domain is not this level's");
@@ -895,6 +904,11 @@ int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc
     }
     L->reg_crse = crse_side;
     L->reg_fine = fine_side;
+    // a coarse level's post_timestep clean follows its reflux and
+    // average-down (Castro::post_timestep, Source/Castro.cpp:343-369): the
+    // fused step of such a level stops after the advance's clean
+    L->defer_post = crse_side != nullptr;
+    if (L->fused) fused_set_defer_post(L->fused, L->defer_post);
     L->fine_index.assign(L->boxes.size(), -1);
     if (fine_side) {
         for (size_t b = 0; b < L->boxes.size(); ++b)
@@ -918,20 +932,19 @@ int mc_level_set_flux_registers(mc_level_t *L, mc_flux_register_t *crse_side, mc
 // Castro::reflux (Source/Castro.cpp:451-486) into this (coarse) level's S_new
 int mc_level_reflux(mc_level_t *L, mc_flux_register_t *R)
 {
-    if (L->mode != MC_MODE_PLAIN) {
-        set_error("mc_level_reflux: per-box path (MC_MODE_PLAIN) only");
-        return -1;
-    }
     for (int d = 0; d < 3; ++d)
         if (R->crse_n[d] != L->n_cell[d]) {
             set_error("mc_level_reflux: the register's coarse domain is not this level's");
             return -1;
         }
-    if (alloc_plain(L)) return -1;
+    // a fused level hands its state to S_new (the engine takes it back at
+    // its next step); the cell volume is Geometry::GetVolume's dx dy dz
+    if (sync_from_fused(L)) return -1;
+    const double vol = L->dx[0] * L->dx[1] * L->dx[2];
     if (flux_register_clear_internal(R, L->stream)) return -1;
     for (size_t b = 0; b < L->boxes.size(); ++b)
-        if (flux_register_reflux(R, L->S_new + L->off_valid[b], L->boxes[b], L->boxes[b], L->vol,
-                                 grow1(L->boxes[b], 4), 1.0, L->stream))
+        if (flux_register_reflux(R, L->S_new + L->off_valid[b], L->boxes[b], L->boxes[b], nullptr, L->boxes[b],
+                                 1.0, L->stream, vol))
             return -1;
     if (flux_register_fill(R, 0.0, L->stream)) return -1;
     return check_hip(hipStreamSynchronize(L->stream), "reflux");

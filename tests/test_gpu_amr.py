@@ -140,30 +140,28 @@ def test_two_level_sedov_matches_oracle_and_conserves():
     assert (Rg.data() == 0).all()
     for c in (0, 4):
         assert abs(S1[..., c].sum() - S0[..., c].sum()) <= 2e-14 * S0[..., c].sum(), c
-    # fused-mode levels refuse registers (the fused path stores no fluxes)
-    Gc.close()
-    F = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mc.MODE_FUSED)
-    with pytest.raises(RuntimeError):
-        amr.set_flux_registers(F, crse_side=Rg)
 
 
-@pytest.mark.parametrize("fine_slab", [0, 5])
-def test_two_level_sedov_with_a_fused_fine_level(fine_slab):
-    """The same two-level step with the fine level on the fused engine: its
-    FineAdd comes from the engine's final fluxes, its average-down from the
-    engine's state, and both levels still equal the oracle bit for bit.  A
-    fused level cannot be a register's coarse side, nor feed one from
-    z-slabs."""
+@pytest.mark.parametrize("crse_mode,fine_mode,fine_slab", [("plain", "fused", 0), ("fused", "plain", 0),
+                                                            ("fused", "fused", 0), ("plain", "fused", 5)])
+def test_two_level_sedov_with_fused_levels(crse_mode, fine_mode, fine_slab):
+    """The same two-level step with either level, or both, on the fused
+    engine: the registers are fed from the engine's final fluxes, a fused
+    coarse level stops its step after the advance's clean_state and takes
+    post_timestep's clean after the reflux and average-down, and both levels
+    still equal the oracle bit for bit.  A register is not fed from z-slabs."""
     from test_oracle_amr import two_level
     nc = 16
     Lc, Lf, R, region, totals = two_level(nc=nc, nthreads=8)
+    mode = {"plain": mc.MODE_PLAIN, "fused": mc.MODE_FUSED}
     cfg_c = small_cfg(n=nc, bc=1, r_init=0.15)
     cfg_c["boxes"] = configs.grid_boxes(nc, 8)
     cfg_f = small_cfg(n=2 * nc, bc=1, r_init=0.15)
     cfg_f["boxes"] = configs.grid_boxes(2 * nc, 8)
-    Gc = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mc.MODE_PLAIN)
-    Gf = mc.Level(cfg_f["n_cell"], cfg_f["boxes"], cfg_f["prob_lo"], cfg_f["prob_hi"], 1, mc.MODE_FUSED)
-    Gf.set_fused_slab(fine_slab)
+    Gc = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mode[crse_mode])
+    Gf = mc.Level(cfg_f["n_cell"], cfg_f["boxes"], cfg_f["prob_lo"], cfg_f["prob_hi"], 1, mode[fine_mode])
+    if fine_slab:
+        Gf.set_fused_slab(fine_slab)
     Gc.init_data(**cfg_c)
     Gf.init_data(**cfg_f)
     Rg = amr.FluxRegister(cfg_f["boxes"][region], RATIO, (nc,) * 3, 1)
@@ -185,6 +183,7 @@ def test_two_level_sedov_with_a_fused_fine_level(fine_slab):
     np.testing.assert_array_equal(Gf.state(), Lf.state())
     np.testing.assert_array_equal(Gc.state(), Lc.state())
     assert (Rg.data() == 0).all()
-    F = mc.Level(cfg_c["n_cell"], cfg_c["boxes"], cfg_c["prob_lo"], cfg_c["prob_hi"], 1, mc.MODE_FUSED)
-    with pytest.raises(RuntimeError):
-        amr.set_flux_registers(F, crse_side=Rg)
+    # a level with registers steps through advance / post_timestep
+    for G in (Gc, Gf):
+        with pytest.raises(RuntimeError):
+            G.run(1)

@@ -278,7 +278,9 @@ int mc_level_fused_slab(mc_level_t *L);
    and finalc (trans1 and final by cells), whose bricks overlap by one cell
    along their direction, take (W, L, X): bricks (W, L, 1) along y,
    (W, 1, L) along z, (X, W*L/X, 1) along x; L, X >= 2, W*L threads
-   (multiple of 64, <= 256), X dividing W*L. */
+   (multiple of 64, <= 256), X dividing W*L.  tflux stages its bricks'
+   primitives in LDS with two cells either side: (L + 4) W <= 448 and
+   (X + 4) W L / X <= 448. */
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
 /* Order in which a stage's bricks are launched: 0 raster (x fastest, then

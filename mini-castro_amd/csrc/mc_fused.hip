@@ -186,13 +186,23 @@ struct StateH {
     double rho, u, v, w, p, rhoe;
 };
 
+// Doubles per component of K_tflux's LDS tile of the primitives: a brick
+// of n threads with b cells along its direction stages (b + 4) n / b cells
+// (the stencil's two cells either side), at most QTILE.
+constexpr int QTILE = 448;
+
 // PPM reconstruction + characteristic tracing (Source/ppm.F90:309-743) of
 // cell (i,j,k) of instance m in direction d, hydro part: the plus state of
 // face pos (stored when own_p and pos >= 0) and the minus state of face
 // pos + 1, both also returned as the Riemann solver of direction d takes
-// them (fp, fm: the values load_fs_h reads back from the stored states)
+// them (fp, fm: the values load_fs_h reads back from the stored states).
+// STAGED: the primitives of cells pos-2 .. pos+2 come from the block's LDS
+// tile (component c of cell pos-2+q at tile[c * QTILE + self + q * sr])
+// instead of the cell arrays.
+template <bool STAGED>
 __device__ __forceinline__ void trace_hydro(const Ctx &x, int m, const FInst &I, int d, int i, int j, int k,
-                                            bool own_p, TraceCtx &t, FaceH &fp, FaceH &fm)
+                                            bool own_p, TraceCtx &t, FaceH &fp, FaceH &fm,
+                                            const double *tile = nullptr, int self = 0, int sr = 0)
 {
     const int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const int pos = d == 0 ? i : (d == 1 ? j : k);
@@ -200,23 +210,27 @@ __device__ __forceinline__ void trace_hydro(const Ctx &x, int m, const FInst &I,
     // the minus state of face pos + 1; in z it must lie in the slab's faces
     t.do_m = pos <= I.n[d] - 1 && (d != 2 || k + 1 <= I.zlo + I.zn);
     const int cun = FQ_U + d, cut = d == 0 ? FQ_V : (d == 1 ? FQ_W : FQ_U), cutt = d == 0 ? FQ_W : (d == 1 ? FQ_U : FQ_V);
-    const idx_t c0 = cidx(I, i, j, k);
-    const double cc = CA(x.Q, I, NFQ, FQ_C, c0);
-    const double un = CA(x.Q, I, NFQ, cun, c0);
+#pragma unroll
+    for (int s = -2; s <= 2; ++s) t.cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
+    // component c of the primitives of cell pos - 2 + q
+    auto qs = [&](int c, int q) -> double {
+        if (STAGED) return tile[c * QTILE + self + q * sr];
+        return CA(x.Q, I, NFQ, c, t.cs[q]);
+    };
+    const double cc = qs(FQ_C, 2);
+    const double un = qs(cun, 2);
     const double flatn = EA(x.FL, I, 1, 0, eidx(I, i, j, k));
     const double dtdx = x.dt_cur[m] / I.dx[d];
     t.un = un;
     t.flatn = flatn;
     t.dtdx = dtdx;
-#pragma unroll
-    for (int s = -2; s <= 2; ++s) t.cs[s + 2] = cidx(I, i + s * di, j + s * dj, k + s * dk);
 
     double Ip[3][4], Im[3][4], Ip_ut = 0, Im_ut = 0, Ip_utt = 0, Im_utt = 0;
     const int vars[6] = {FQ_RHO, cun, FQ_P, FQ_RE, cut, cutt};
     for (int vv = 0; vv < 6; ++vv) {
         double s[5], sm_, sp_;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, vars[vv], t.cs[q]);
+        for (int q = 0; q < 5; ++q) s[q] = qs(vars[vv], q);
         ppm_reconstruct(s, flatn, sm_, sp_);
         const double s6 = ppm_s6(sm_, sp_, s[2]);
         if (vv < 4) {
@@ -233,7 +247,7 @@ __device__ __forceinline__ void trace_hydro(const Ctx &x, int m, const FInst &I,
     {
         double s[5], sm_, sp_, a, b;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] = CA(x.Q, I, NFQ, FQ_G, t.cs[q]);
+        for (int q = 0; q < 5; ++q) s[q] = qs(FQ_G, q);
         ppm_reconstruct(s, flatn, sm_, sp_);
         const double s6 = ppm_s6(sm_, sp_, s[2]);
         ppm_int_wave(sm_, sp_, s6, un - cc, dtdx, a, gam_m);
@@ -320,7 +334,7 @@ __global__ __launch_bounds__(256) void K_trace(Plan p, Ctx x)
     if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
     TraceCtx t;
     FaceH fp, fm;
-    trace_hydro(x, m, p.inst[m], sub_, i, j, k, true, t, fp, fm);
+    trace_hydro<false>(x, m, p.inst[m], sub_, i, j, k, true, t, fp, fm);
     // species last: their stores come after every read of the instance table
     for (int pr = 0; pr < NSPEC_PAIRS; ++pr) {
         dbl2 xp, xm;
@@ -372,26 +386,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void K
 __device__ __forceinline__ int below_of(int tid, int st) { return tid - st; }
 
 // K_trace and K_flux1 in one launch.  The bricks of direction d overlap
-// along d by one cell layer (plan_cell_ovl).  Every thread traces its cell
-// and stores both states as K_trace does; the minus state (for the cell's
-// high d-face) also goes to LDS, and after the block's barrier the thread
-// of each layer l >= 1 solves the Riemann problem on its cell's low d-face
-// with the minus state the layer below left in LDS and its own plus state,
-// kept in registers: the first-stage flux reads no traced state from
-// memory.  The species follow the same way, all pairs at once.
+// along d by one cell layer (plan_cell_ovl).  The block first copies the
+// primitives its stencils reach (its brick and two cells either side along
+// d, every component) into an LDS tile, one global load per cell and
+// component; every thread then traces its cell from the tile and stores
+// both states as K_trace does.  The minus state (for the cell's high d-face)
+// also goes to LDS, and after the block's barrier the thread of each layer
+// l >= 1 solves the Riemann problem on its cell's low d-face with the minus
+// state the layer below left in LDS and its own plus state, kept in
+// registers, and the sound speeds and Gamma_1 of the tile: the first-stage
+// flux reads no traced state from memory.  The species follow the same way,
+// all pairs at once.
 // A layer-0 cell belongs to the brick below as its top layer: it stores its
 // minus state (the same values that brick stores) but not its plus state,
 // and solves no Riemann problem.  The sub-launches are the directions,
 // interleaved block by block like the other face stages.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K_tflux(Plan3 p3, Ctx x)
 {
-    // the minus states' hydro part, then (after the barrier that ends its
-    // reads) their species in the same memory
+    // the minus states' hydro part; the primitives' tile, then (after the
+    // barrier that ends its reads) the minus states' species in its memory
+    __shared__ FaceH lds_h[256];
     __shared__ union {
-        FaceH h[256];
+        double q[NFQ * QTILE];
         dbl2 s[NSPEC_PAIRS][256];
     } lds;
-    FaceH(&lds_h)[256] = lds.h;
+    double *tile = lds.q;
     dbl2(&lds_s)[NSPEC_PAIRS][256] = lds.s;
     int lb = logical_block(), d;
     if (gridDim.y > 1) {
@@ -409,10 +428,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K
     const int below = tid - (d == 0 ? 1 : (d == 1 ? p.bx : p.bx * p.by));  // the thread of layer l - 1
     const int pos = d == 0 ? i : (d == 1 ? j : k);
     const bool fx = ok && l > 0 && pos >= lo_index(I, d);
+    // the tile: bd + 4 rows along d (row r: cell pos0 - 2 + r of the brick's
+    // layer-0 position pos0) of cw cells across; tile element e is row
+    // e % (bd + 4), column e / (bd + 4) in x (rows contiguous, as the x
+    // bricks' lanes run along x) and row e / cw, column e % cw in y and z
+    // (columns contiguous, x fastest), so that both the global loads and the
+    // threads' LDS reads are consecutive across lanes
+    const int nthr = p.bx * p.by * p.bz;
+    const int bd = d == 0 ? p.bx : (d == 1 ? p.by : p.bz);
+    const int cw = nthr / bd, rows = bd + 4;
+    const int ti = tid % p.bx, tj = (tid / p.bx) % p.by, tk = tid / (p.bx * p.by);
+    const int i0 = i - ti, j0 = j - tj, k0 = k - tk;  // the brick's first cell
+    for (int e = tid; e < rows * cw; e += nthr) {
+        const int r = d == 0 ? e % rows : e / cw, c = d == 0 ? e / rows : e % cw;
+        int a, b, cz;
+        if (d == 0) {
+            a = i0 - 2 + r;
+            b = j0 + c % p.by;
+            cz = k0 + c / p.by;
+        } else if (d == 1) {
+            a = i0 + c % p.bx;
+            b = j0 - 2 + r;
+            cz = k0 + c / p.bx;
+        } else {
+            a = i0 + c % p.bx;
+            b = j0 + c / p.bx;
+            cz = k0 - 2 + r;
+        }
+        const idx_t ce = cidx(I, a, b, cz);
+#pragma unroll
+        for (int n = 0; n < NFQ; ++n) tile[n * QTILE + e] = CA(x.Q, I, NFQ, n, ce);
+    }
+    const int sr = d == 0 ? 1 : cw;  // tile stride of one row
+    const int self = d == 0 ? l + (tj + p.by * tk) * rows : l * cw + (d == 1 ? ti + p.bx * tk : ti + p.bx * tj);
+    __syncthreads();
     TraceCtx t;
     FaceH fp, fm;
     if (ok) {
-        trace_hydro(x, m, I, d, i, j, k, l > 0, t, fp, fm);
+        trace_hydro<true>(x, m, I, d, i, j, k, l > 0, t, fp, fm, tile, self, sr);
         lds_h[tid] = fm;
     }
     const SpeciesPairs XF = species_pairs(x.FD[d], I, NFF, FF_X);
@@ -421,13 +474,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K
     bool useL = false, useR = false;
     if (fx) {
         const FaceH L = lds_h[below];
-        face_riemann_h(x, I, d, i, j, k, L, fp, o);
+        // cells pos - 1 and pos: tile offsets 1 and 2 from the thread's row
+        riemann_hydro(L, fp, tile[FQ_C * QTILE + self + sr], tile[FQ_C * QTILE + self + 2 * sr],
+                      tile[FQ_G * QTILE + self + sr], tile[FQ_G * QTILE + self + 2 * sr], o);
         store_tflux_h(x.FD[d], I, t.ep, d, o);
         // a side whose weight is zero contributes +-0 to fpl*xl + fmr*xr
         useL = o.fpl != 0.0;
         useR = o.fmr != 0.0;
     }
-    __syncthreads();  // the hydro minus states are read: LDS takes the species
+    __syncthreads();  // the tile is read: its memory takes the species
     // every pair's species trace first (the minus states to LDS, the plus
     // states kept), so that their loads are in flight together; one barrier
     dbl2 xpv[NSPEC_PAIRS];
@@ -1763,10 +1818,24 @@ void fused_set_defer_post(FusedLevel *F, bool defer)
     F->graph = nullptr;
 }
 
+// K_tflux's bricks must fit their primitives' tile (bd + 4 rows of the
+// brick's cells across, at most QTILE)
+static bool tflux_tile_fits(const KernelTiling &t)
+{
+    for (int d = 0; d < 3; ++d) {
+        const KernelTiling b = ovl_brick(t, d);
+        const int bd = d == 0 ? b.bx : (d == 1 ? b.by : b.bz);
+        if ((bd + 4) * (b.bx * b.by * b.bz / bd) > QTILE) return false;
+    }
+    return true;
+}
+
 bool fused_tiling_ok(int stage, int bx, int by, int bz)
 {
     if (stage < 0 || stage >= NSTAGE) return false;
-    if (overlap_stage(stage)) return ovl_tiling_ok(KernelTiling{bx, by, bz});
+    if (overlap_stage(stage))
+        return ovl_tiling_ok(KernelTiling{bx, by, bz}) &&
+               (stage != ST_TFLUX || tflux_tile_fits(KernelTiling{bx, by, bz}));
     return bx > 0 && by > 0 && bz > 0 && (bx * by * bz) % 64 == 0 && bx * by * bz <= 256;
 }
 
@@ -1930,6 +1999,7 @@ int fused_autotune(FusedLevel *F, int reps, hipStream_t s, int *bricks, double *
             return ms / reps;
         };
         for (const KernelTiling &c : cands) {
+            if (!fused_tiling_ok(st, c.bx, c.by, c.bz)) continue;
             const double t = time_tiling(c);
             if (t < 0) break;
             if (t < tbest) {

@@ -447,7 +447,8 @@ int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
     if (!fused_tiling_ok(stage, bx, by, bz)) {
         set_error(stage == FST_TFLUX || stage == FST_TRANS1C || stage == FST_FINALC
                       ? "mc_level_set_tiling: the tflux / trans1c / finalc tiling (W, L, X) needs L, X >= 2, W*L a "
-                        "multiple of 64 threads, at most 256, and X dividing W*L"
+                        "multiple of 64 threads, at most 256, and X dividing W*L; tflux also (L + 4) W <= 448 and "
+                        "(X + 4) W L / X <= 448 (its LDS tile)"
                       : "mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256");
         return -1;
     }

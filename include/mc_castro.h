@@ -219,7 +219,9 @@ void mc_set_problem(double p_ambient, double dens_ambient, double exp_energy, do
    MC_MODE_PLAIN replays the reference's per-box kernel sequence through the
    Part 1 entry points; MC_MODE_FUSED advances all boxes with one launch per
    stage (bit-identical results).  In fused mode mc_level_advance includes
-   post_timestep's clean_state and mc_level_post_timestep does nothing. */
+   post_timestep's clean_state and mc_level_post_timestep does nothing,
+   except on the coarse side of a flux register (mc_level_set_flux_registers),
+   whose post_timestep clean follows the reflux and average-down. */
 typedef struct mc_level mc_level_t;
 enum { MC_BC_PERIODIC = 0, MC_BC_OUTFLOW = 1 };
 enum { MC_MODE_PLAIN = 0, MC_MODE_FUSED = 1 };

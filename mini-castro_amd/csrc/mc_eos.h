@@ -14,7 +14,10 @@
 //    at the temperature the reference's final pass uses.  Every quantity that
 //    is computed is computed with the reference's expression, so results are
 //    bit-identical to the reference arithmetic when built without FMA
-//    contraction.
+//    contraction;
+//  * the h5 sums of one evaluation run together, two table columns at a
+//    time (each sum keeps its own term order), so the four nodes' columns
+//    are not all held in registers at once.
 // The electron chemical potential and number density tables (ef*, xf*) and
 // the entropy terms feed nothing on the hydro path and are not evaluated.
 #pragma once
@@ -143,6 +146,23 @@ __device__ __forceinline__ double h3_dpdf(const Nodes &N, double t0, double t1, 
 }
 #undef MC_H5_TERMS
 
+// One column of a biquintic or bicubic sum: the four nodes' values a b c d
+// (nodes (i,j) (i+1,j) (i,j+1) (i+1,j+1)) with the density weights (wd, wmd)
+// and temperature weights (wt, wmt), added to acc in h5's term order (the
+// first column starts the sum).
+__device__ __forceinline__ void hcol(double &acc, bool first, double a, double b, double c, double d, double wd,
+                                     double wmd, double wt, double wmt)
+{
+    acc = first ? a * wd * wt : acc + a * wd * wt;
+    acc = acc + b * wmd * wt;
+    acc = acc + c * wd * wmt;
+    acc = acc + d * wmd * wmt;
+}
+
+// weight k (0, 1, 2) of a W6, and its m counterpart
+__device__ __forceinline__ double wsel(const W6 &w, int k) { return k == 0 ? w.w0 : (k == 1 ? w.w1 : w.w2); }
+__device__ __forceinline__ double wselm(const W6 &w, int k) { return k == 0 ? w.w0m : (k == 1 ? w.w1m : w.w2m); }
+
 // Density-side data of one EOS call (constant through the Newton solve).
 struct EosDen {
     double den, deni, ytot1, ye, din;
@@ -218,13 +238,10 @@ __device__ __forceinline__ void eos_eval(const EosTable &tab, const EosDen &s, d
     const double *g = tab.tg + (long)(jat - 1) * EOS_GRID;
     const double tJ = g[G_X], dt = g[G_D], dt2 = g[G_D2], dti = g[G_DI], dt2i = g[G_D2I];
 
-    Nodes N;
     const double *n0 = tab.node + ((long)(s.iat - 1) + (long)EOS_IMAX * (jat - 1)) * EOS_NODE;
-    constexpr int ncol = FULL ? 13 : 9;
-    load_node(n0, N.a, ncol);
-    load_node(n0 + EOS_NODE, N.b, ncol);
-    load_node(n0 + (long)EOS_IMAX * EOS_NODE, N.c, ncol);
-    load_node(n0 + (long)EOS_IMAX * EOS_NODE + EOS_NODE, N.d, ncol);
+    const double2 *qa = reinterpret_cast<const double2 *>(n0), *qb = reinterpret_cast<const double2 *>(n0 + EOS_NODE),
+                  *qc = reinterpret_cast<const double2 *>(n0 + (long)EOS_IMAX * EOS_NODE),
+                  *qd = reinterpret_cast<const double2 *>(n0 + (long)EOS_IMAX * EOS_NODE + EOS_NODE);
 
     const double xt = fmax((temp - tJ) * dti, 0.0);
     const double mxt = 1.0 - xt;
@@ -249,9 +266,32 @@ __device__ __forceinline__ void eos_eval(const EosTable &tab, const EosDen &s, d
     ddT.w1m = -ddpsi1(mxt) * dti;
     ddT.w2m = ddpsi2(mxt);
 
-    const double free = h5(N, T, s.D);
-    const double df_t = h5(N, dT, s.D);
-    const double df_tt = h5(N, ddT, s.D);
+    // The biquintic sums (Source/eos.F90:915-950) of the free energy and its
+    // derivatives, column by column: every sum adds the same terms in the
+    // same order as h5, but only two columns of the four nodes are live at a
+    // time.  Column q takes density weight dsel and temperature weight tsel.
+    constexpr int dsel[9] = {0, 0, 0, 1, 2, 1, 2, 1, 2};
+    constexpr int tsel[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
+    double free = 0.0, df_t = 0.0, df_tt = 0.0, df_d = 0.0, df_dt = 0.0;
+#pragma unroll
+    for (int c2 = 0; c2 < 5; ++c2) {
+        const double2 va = qa[c2], vb = qb[c2], vc = qc[c2], vd = qd[c2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = 2 * c2 + h;
+            if (q >= 9) break;
+            const double a = h ? va.y : va.x, b = h ? vb.y : vb.x, c = h ? vc.y : vc.x, d = h ? vd.y : vd.x;
+            const double wd = wsel(s.D, dsel[q]), wmd = wselm(s.D, dsel[q]);
+            hcol(free, q == 0, a, b, c, d, wd, wmd, wsel(T, tsel[q]), wselm(T, tsel[q]));
+            hcol(df_t, q == 0, a, b, c, d, wd, wmd, wsel(dT, tsel[q]), wselm(dT, tsel[q]));
+            hcol(df_tt, q == 0, a, b, c, d, wd, wmd, wsel(ddT, tsel[q]), wselm(ddT, tsel[q]));
+            if (FULL) {
+                const double ed = wsel(s.dD, dsel[q]), emd = wselm(s.dD, dsel[q]);
+                hcol(df_d, q == 0, a, b, c, d, ed, emd, wsel(T, tsel[q]), wselm(T, tsel[q]));
+                hcol(df_dt, q == 0, a, b, c, d, ed, emd, wsel(dT, tsel[q]), wselm(dT, tsel[q]));
+            }
+        }
+    }
 
     const double sele = -df_t * ye;
     const double dsepdt = -df_tt * ye;
@@ -262,10 +302,19 @@ __device__ __forceinline__ void eos_eval(const EosTable &tab, const EosDen &s, d
     o.dedT = deraddt + deiondt + deepdt;
 
     if (FULL) {
-        const double df_d = h5(N, T, s.dD);
-        const double df_dt = h5(N, dT, s.dD);
         const double xt0 = xpsi0(xt), xt1 = xpsi1(xt) * dt, xt0m = xpsi0(mxt), xt1m = -xpsi1(mxt) * dt;
-        double dpepdd = h3_dpdf(N, xt0, xt1, xt0m, xt1m, s.x0, s.x1, s.x0m, s.x1m);
+        // the bicubic dpdf sum (Source/eos.F90:1034-1059), columns 9..12 in
+        // h3_dpdf's order: (d0, t0), (d0, t1), (d1, t0), (d1, t1)
+        double dpepdd = 0.0;
+        {
+            const double2 a0 = qa[4], b0 = qb[4], c0 = qc[4], d0 = qd[4];  // column 9 in .y
+            hcol(dpepdd, true, a0.y, b0.y, c0.y, d0.y, s.x0, s.x0m, xt0, xt0m);
+            const double2 a1 = qa[5], b1 = qb[5], c1 = qc[5], d1 = qd[5];  // columns 10, 11
+            hcol(dpepdd, false, a1.x, b1.x, c1.x, d1.x, s.x0, s.x0m, xt1, xt1m);
+            hcol(dpepdd, false, a1.y, b1.y, c1.y, d1.y, s.x1, s.x1m, xt0, xt0m);
+            const double2 a2 = qa[6], b2 = qb[6], c2 = qc[6], d2 = qd[6];  // column 12 in .x
+            hcol(dpepdd, false, a2.x, b2.x, c2.x, d2.x, s.x1, s.x1m, xt1, xt1m);
+        }
         dpepdd = fmax(ye * dpepdd, 0.0);
 
         const double deraddd = -erad * deni;

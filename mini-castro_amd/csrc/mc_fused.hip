@@ -508,18 +508,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K
 
 // trans1 (Source/trans.F90:14-243) of the t-face states by the d-flux, then
 // the flux F^{t|d} of the corrected states on the t-faces of tbx[t][d].
+// The hydro part of the d-fluxes the brick's faces read (on the d-faces of
+// the cells either side of each t-face: the brick and one cell more below
+// along t and above along d) is first copied into LDS, one global load per
+// face and pair of components; the block's LDS is sized by the launch
+// (trans1_tile_cells).
 // Held to 5 waves per SIMD (96 VGPRs, no spills): 12.50 against 13.02 ms
 // at the 4 waves its 99 VGPRs would give.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void K_trans1(Plan p, Ctx x)
 {
+    extern __shared__ double ftile[];  // 7 components x nT cells
     int m, i, j, k, sub_;
-    if (!plan_cell_sub(p, 6, sub_, m, i, j, k)) return;
+    const bool inr = plan_cell_sub(p, 6, sub_, m, i, j, k);
     const int combo = sub_;
     const int t = c_pair_t[combo], d = c_pair_d[combo];
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
     // range: t-face 0..n_t, d-cells 0..n_d-1, the third direction -1..n
-    if (c[t] < lo_index(I, t) || c[d] < lo_index(I, d) || c[d] > hi_cell(I, d)) return;
+    const bool act = inr && !(c[t] < lo_index(I, t) || c[d] < lo_index(I, d) || c[d] > hi_cell(I, d));
+    // the d-flux tile: the brick's faces widened by one below along t and one
+    // above along d, x fastest; cell (ax, ay, az) is face (i0, j0, k0) + (ax,
+    // ay, az) - e_t of the extended d-face array
+    const int tid = threadIdx.x, nthr = p.bx * p.by * p.bz;
+    const int ti = tid % p.bx, tj = (tid / p.bx) % p.by, tk = tid / (p.bx * p.by);
+    const int et0 = t == 0, et1 = t == 1, et2 = t == 2, ed0 = d == 0, ed1 = d == 1, ed2 = d == 2;
+    const int Tx = p.bx + et0 + ed0, Ty = p.by + et1 + ed1, Tz = p.bz + et2 + ed2, nT = Tx * Ty * Tz;
+    {
+        double *FDd = x.FD[d];
+        const int gx0 = i - ti - et0, gy0 = j - tj - et1, gz0 = k - tk - et2;
+        for (int e = tid; e < nT; e += nthr) {
+            const int gx = gx0 + e % Tx, gy = gy0 + (e / Tx) % Ty, gz = gz0 + e / (Tx * Ty);
+            // only faces of the extended array (the brick may reach past it)
+            if (gx < -1 || gx > I.n[0] || gy < -1 || gy > I.n[1] || gz < I.zlo - 1 || gz > I.zlo + I.zn) continue;
+            const idx_t ge = eidx(I, gx, gy, gz);
+            const dbl2 a = EF2(FDd, I, NFF, FF_RHO, ge), b = EF2(FDd, I, NFF, FF_MY, ge), cc = EF2(FDd, I, NFF, FF_E, ge);
+            ftile[e] = a.x;
+            ftile[nT + e] = a.y;
+            ftile[2 * nT + e] = b.x;
+            ftile[3 * nT + e] = b.y;
+            ftile[4 * nT + e] = cc.x;
+            ftile[5 * nT + e] = cc.y;
+            ftile[6 * nT + e] = EF(FDd, I, NFF, FF_GU, ge);
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    // the tile's copy of load_tflux_h at tile cell q
+    auto tile_tflux = [&](int q, double *v, double &gp, double &gu) {
+        v[URHO] = ftile[q];
+        v[UMX] = ftile[nT + q];
+        v[UMY] = ftile[2 * nT + q];
+        v[UMZ] = ftile[3 * nT + q];
+        v[UEDEN] = ftile[4 * nT + q];
+        gp = ftile[5 * nT + q];
+        gu = ftile[6 * nT + q];
+    };
     const idx_t e = eidx(I, i, j, k);
     const int et[3] = {t == 0, t == 1, t == 2}, ed[3] = {d == 0, d == 1, d == 2};
     const double cdtdx = x.dt_cur[m] / I.dx[d];
@@ -536,8 +579,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void K
         QInH lq;
         load_qin_h(s == 0 ? x.SM[t] : x.SP[t], I, e, lq);
         double fr[UEDEN + 1], fl[UEDEN + 1], pgp, ugp, pgm, ugm;
-        load_tflux_h(x.FD[d], I, er[s], fr, pgp, ugp);
-        load_tflux_h(x.FD[d], I, el[s], fl, pgm, ugm);
+        // the side's cell in the tile: the thread's face + s e_t (the tile
+        // starts one cell below along t), and its high d-face one above
+        const int ql = (ti + s * et0) + Tx * ((tj + s * et1) + Ty * (tk + s * et2));
+        tile_tflux(ql + (ed0 + Tx * (ed1 + Ty * ed2)), fr, pgp, ugp);
+        tile_tflux(ql, fl, pgm, ugm);
         const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, il, jl, kl));
         const QStateH o = trans1_hydro(lq, fr, fl, pgp, pgm, ugp, ugm, gamc, cdt);
         rho[s] = lq.rho;
@@ -1202,6 +1248,21 @@ Plan get_plan(FusedLevel *F, int stage, int slab)
     return plan_for(F, F->tiling[stage], stage_shift[stage], stage_add[stage], slab, -1);
 }
 
+// K_trans1's LDS for a brick: its d-flux tile (the brick widened by one cell
+// along t and one along d) of 7 components, for the largest of the six pairs
+static size_t trans1_tile_bytes(int bx, int by, int bz)
+{
+    static const int pt[6] = {1, 2, 0, 2, 0, 1}, pd[6] = {0, 0, 1, 1, 2, 2};
+    long cells = 0;
+    for (int c = 0; c < 6; ++c) {
+        long n = 1;
+        const int b[3] = {bx, by, bz};
+        for (int a = 0; a < 3; ++a) n *= b[a] + (a == pt[c]) + (a == pd[c]);
+        cells = std::max(cells, n);
+    }
+    return (size_t)cells * 7 * sizeof(double);
+}
+
 // the three direction plans of an overlapping stage
 static Plan3 get_plan3(FusedLevel *F, int stage, int slab)
 {
@@ -1257,7 +1318,7 @@ static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1, in
     case ST_FLAT: hipLaunchKernelGGL(K_flat, g, b, 0, s, p, F->ctx); break;
     case ST_TRACE: hipLaunchKernelGGL(K_trace, g, b, 0, s, p, F->ctx); break;
     case ST_FLUX1: hipLaunchKernelGGL(K_flux1, g, b, 0, s, p, F->ctx); break;
-    case ST_TRANS1: hipLaunchKernelGGL(K_trans1, g, b, 0, s, p, F->ctx); break;
+    case ST_TRANS1: hipLaunchKernelGGL(K_trans1, g, b, trans1_tile_bytes(p.bx, p.by, p.bz), s, p, F->ctx); break;
     case ST_FINAL: hipLaunchKernelGGL(K_final, g, b, 0, s, p, F->ctx); break;
     case ST_UPDATE:
         if (F->defer_post) hipLaunchKernelGGL(K_update_adv, g, b, 0, s, p, F->ctx);

@@ -466,7 +466,7 @@ def main():
         L.set_stage_variant(prev["config"].get("stage_variant", 0))
         tuned = None
     else:
-        tuned = L.autotune(reps=2)   # restores the state it times on; picks the kernel variant
+        tuned = L.autotune(reps=4)   # restores the state it times on; picks the kernel variant
         log("autotuned")
     if a.stage_variant >= 0:
         L.set_stage_variant(a.stage_variant)

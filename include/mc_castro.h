@@ -282,9 +282,8 @@ int mc_level_fused_slab(mc_level_t *L);
    (W, 1, L) along z, (X, W*L/X, 1) along x; L, X >= 2, W*L threads
    (multiple of 64, <= 256), X dividing W*L.  tflux stages its bricks'
    primitives in LDS with two cells either side: (L + 4) W <= 448 and
-   (X + 4) W L / X <= 448.  trans1 and final stage their transverse fluxes
-   in LDS (the brick widened by one cell in two directions, 7 doubles a
-   cell; two such tiles for final), at most 64 KB. */
+   (X + 4) W L / X <= 448.  trans1 stages its d-fluxes in LDS (the brick
+   widened by one cell in two directions, 7 doubles a cell), at most 64 KB. */
 int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz);
 int mc_level_get_tiling(mc_level_t *L, int stage, int *bxyz);
 /* Order in which a stage's bricks are launched: 0 raster (x fastest, then

@@ -813,67 +813,21 @@ __device__ __forceinline__ void final_species_store(const SpeciesPairs &XN, doub
 // on the d-faces of the valid region.
 __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
 {
-    extern __shared__ double ftile[];  // the F^{t1|t2} tile, then the F^{t2|t1} tile (7 components each)
     int m, i, j, k, sub_;
-    const bool inr = plan_cell_sub(p, 3, sub_, m, i, j, k);
+    if (!plan_cell_sub(p, 3, sub_, m, i, j, k)) return;
     const int d = sub_;
     const FInst &I = p.inst[m];
     const int c[3] = {i, j, k};
-    bool act = inr;
     for (int a = 0; a < 3; ++a) {
-        if (c[a] < lo_index(I, a)) act = false;
-        if (a != d && c[a] > hi_cell(I, a)) act = false;
+        if (c[a] < lo_index(I, a)) return;
+        if (a != d && c[a] > hi_cell(I, a)) return;
     }
     const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
     const int ed[3] = {d == 0, d == 1, d == 2}, e1[3] = {t1 == 0, t1 == 1, t1 == 2}, e2[3] = {t2 == 0, t2 == 1, t2 == 2};
-    double *F2 = x.FT[pair_index(t1, t2)], *F3 = x.FT[pair_index(t2, t1)];
-    // The hydro part of the two transverse fluxes the brick's d-faces read
-    // (on the t1- and t2-faces of the cells either side of each face): the
-    // brick widened by one cell below along d and one above along t1 (F2) or
-    // t2 (F3), copied into LDS, one global load per face and pair of
-    // components; the launch sizes the LDS for its brick (final_tile_bytes).
-    const int tid = threadIdx.x, nthr = p.bx * p.by * p.bz;
-    const int ti = tid % p.bx, tj = (tid / p.bx) % p.by, tk = tid / (p.bx * p.by);
-    const int A2x = p.bx + ed[0] + e1[0], A2y = p.by + ed[1] + e1[1], n2 = A2x * A2y * (p.bz + ed[2] + e1[2]);
-    const int A3x = p.bx + ed[0] + e2[0], A3y = p.by + ed[1] + e2[1], n3 = A3x * A3y * (p.bz + ed[2] + e2[2]);
-    double *tile2 = ftile, *tile3 = ftile + 7 * n2;
-    {
-        const int gx0 = i - ti - ed[0], gy0 = j - tj - ed[1], gz0 = k - tk - ed[2];
-        auto stage = [&](double *F, double *tl, int Ax, int Ay, int n) {
-            for (int q = tid; q < n; q += nthr) {
-                const int gx = gx0 + q % Ax, gy = gy0 + (q / Ax) % Ay, gz = gz0 + q / (Ax * Ay);
-                // only faces of the extended array (the brick may reach past it)
-                if (gx < -1 || gx > I.n[0] || gy < -1 || gy > I.n[1] || gz < I.zlo - 1 || gz > I.zlo + I.zn)
-                    continue;
-                const idx_t ge = eidx(I, gx, gy, gz);
-                const dbl2 a = EF2(F, I, NFF, FF_RHO, ge), b = EF2(F, I, NFF, FF_MY, ge), cc = EF2(F, I, NFF, FF_E, ge);
-                tl[q] = a.x;
-                tl[n + q] = a.y;
-                tl[2 * n + q] = b.x;
-                tl[3 * n + q] = b.y;
-                tl[4 * n + q] = cc.x;
-                tl[5 * n + q] = cc.y;
-                tl[6 * n + q] = EF(F, I, NFF, FF_GU, ge);
-            }
-        };
-        stage(F2, tile2, A2x, A2y, n2);
-        stage(F3, tile3, A3x, A3y, n3);
-    }
-    __syncthreads();
-    if (!act) return;
-    // the tile's copy of load_tflux_h at tile cell q of a tile of n cells
-    auto tile_tflux = [&](const double *tl, int n, int q, double *v, double &gp, double &gu) {
-        v[URHO] = tl[q];
-        v[UMX] = tl[n + q];
-        v[UMY] = tl[2 * n + q];
-        v[UMZ] = tl[3 * n + q];
-        v[UEDEN] = tl[4 * n + q];
-        gp = tl[5 * n + q];
-        gu = tl[6 * n + q];
-    };
     const double dt = x.dt_cur[m];
     const double h2 = 0.5 * dt / I.dx[t1], h3 = 0.5 * dt / I.dx[t2];
     const idx_t e = eidx(I, i, j, k);
+    double *F2 = x.FT[pair_index(t1, t2)], *F3 = x.FT[pair_index(t2, t1)];
     FaceH side[2];
     idx_t l2[2], r2[2], r3[2];
     double rho[2], rrnew[2];
@@ -888,14 +842,10 @@ __global__ __launch_bounds__(256) void K_final(Plan p, Ctx x)
         load_qin_h(s == 0 ? x.SM[d] : x.SP[d], I, e, lq);
         double f2r[UEDEN + 1], f2l[UEDEN + 1], f3r[UEDEN + 1], f3l[UEDEN + 1], pg2p, ug2p, pg2m, ug2m, pg3p, ug3p,
             pg3m, ug3m;
-        // the side's cell in the tiles: the thread's face + s e_d (the tiles
-        // start one cell below along d), and its high t1- / t2-face
-        const int lx = ti + s * ed[0], ly = tj + s * ed[1], lz = tk + s * ed[2];
-        const int q2 = lx + A2x * (ly + A2y * lz), q3 = lx + A3x * (ly + A3y * lz);
-        tile_tflux(tile2, n2, q2 + (e1[0] + A2x * (e1[1] + A2y * e1[2])), f2r, pg2p, ug2p);
-        tile_tflux(tile2, n2, q2, f2l, pg2m, ug2m);
-        tile_tflux(tile3, n3, q3 + (e2[0] + A3x * (e2[1] + A3y * e2[2])), f3r, pg3p, ug3p);
-        tile_tflux(tile3, n3, q3, f3l, pg3m, ug3m);
+        load_tflux_h(F2, I, r2[s], f2r, pg2p, ug2p);
+        load_tflux_h(F2, I, l2[s], f2l, pg2m, ug2m);
+        load_tflux_h(F3, I, r3[s], f3r, pg3p, ug3p);
+        load_tflux_h(F3, I, l2[s], f3l, pg3m, ug3m);
         const double gamc = CA(x.Q, I, NFQ, FQ_G, cidx(I, il, jl, kl));
         const QStateH o = trans2_hydro(lq, f2r, f2l, f3r, f3l, pg2p, pg2m, ug2p, ug2m, pg3p, pg3m, ug3p, ug3m, gamc,
                                        h2, h3);
@@ -1313,25 +1263,6 @@ static size_t trans1_tile_bytes(int bx, int by, int bz)
     return (size_t)cells * 7 * sizeof(double);
 }
 
-// K_final's LDS for a brick: its two transverse-flux tiles (the brick
-// widened by one cell along d and one along t1, or t2) of 7 components, for
-// the largest of the three directions
-static size_t final_tile_bytes(int bx, int by, int bz)
-{
-    long cells = 0;
-    for (int d = 0; d < 3; ++d) {
-        const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-        const int b[3] = {bx, by, bz};
-        long n2 = 1, n3 = 1;
-        for (int a = 0; a < 3; ++a) {
-            n2 *= b[a] + (a == d) + (a == t1);
-            n3 *= b[a] + (a == d) + (a == t2);
-        }
-        cells = std::max(cells, n2 + n3);
-    }
-    return (size_t)cells * 7 * sizeof(double);
-}
-
 // the three direction plans of an overlapping stage
 static Plan3 get_plan3(FusedLevel *F, int stage, int slab)
 {
@@ -1388,7 +1319,7 @@ static void launch_stage(FusedLevel *F, int stage, hipStream_t s, int ny = 1, in
     case ST_TRACE: hipLaunchKernelGGL(K_trace, g, b, 0, s, p, F->ctx); break;
     case ST_FLUX1: hipLaunchKernelGGL(K_flux1, g, b, 0, s, p, F->ctx); break;
     case ST_TRANS1: hipLaunchKernelGGL(K_trans1, g, b, trans1_tile_bytes(p.bx, p.by, p.bz), s, p, F->ctx); break;
-    case ST_FINAL: hipLaunchKernelGGL(K_final, g, b, final_tile_bytes(p.bx, p.by, p.bz), s, p, F->ctx); break;
+    case ST_FINAL: hipLaunchKernelGGL(K_final, g, b, 0, s, p, F->ctx); break;
     case ST_UPDATE:
         if (F->defer_post) hipLaunchKernelGGL(K_update_adv, g, b, 0, s, p, F->ctx);
         else hipLaunchKernelGGL(K_update, g, b, 0, s, p, F->ctx);
@@ -1967,9 +1898,8 @@ bool fused_tiling_ok(int stage, int bx, int by, int bz)
         return ovl_tiling_ok(KernelTiling{bx, by, bz}) &&
                (stage != ST_TFLUX || tflux_tile_fits(KernelTiling{bx, by, bz}));
     if (!(bx > 0 && by > 0 && bz > 0 && (bx * by * bz) % 64 == 0 && bx * by * bz <= 256)) return false;
-    // the LDS tiles of K_trans1 and K_final stay within a workgroup's 64 KB
+    // K_trans1's LDS tile stays within a workgroup's 64 KB
     if (stage == ST_TRANS1) return trans1_tile_bytes(bx, by, bz) <= 65536;
-    if (stage == ST_FINAL) return final_tile_bytes(bx, by, bz) <= 65536;
     return true;
 }
 

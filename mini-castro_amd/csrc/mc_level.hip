@@ -449,9 +449,9 @@ int mc_level_set_tiling(mc_level_t *L, int stage, int bx, int by, int bz)
                       ? "mc_level_set_tiling: the tflux / trans1c / finalc tiling (W, L, X) needs L, X >= 2, W*L a "
                         "multiple of 64 threads, at most 256, and X dividing W*L; tflux also (L + 4) W <= 448 and "
                         "(X + 4) W L / X <= 448 (its LDS tile)"
-                      : "mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256; trans1 and final "
-                        "bricks also keep their LDS tiles (the brick widened by one cell in two directions, 7 "
-                        "doubles a cell, two tiles for final) within 64 KB");
+                      : "mc_level_set_tiling: tile must be a multiple of 64 threads, at most 256; trans1 bricks "
+                        "also keep their LDS tile (the brick widened by one cell in two directions, 7 doubles a "
+                        "cell) within 64 KB");
         return -1;
     }
     fused_set_tiling(L->fused, stage, bx, by, bz);

@@ -93,8 +93,6 @@ def test_tiling_does_not_change_results():
         B.set_tiling(8, 64, 1, 64)      # L >= 2
     with pytest.raises(RuntimeError):
         B.set_brick_order(6, 1)
-    with pytest.raises(RuntimeError):
-        B.set_tiling(6, 128, 2, 1)      # the final stage's LDS tiles exceed 64 KB
 
 
 def test_autotune_keeps_state_and_results():
